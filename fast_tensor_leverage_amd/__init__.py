@@ -1,0 +1,12 @@
+"""MI355X (gfx950) implementation of exact Khatri-Rao leverage-score sampling
+and sampled CP-ALS, with the surface of fast_tensor_leverage's
+``cpp_ext.als_module``.
+
+The compute path is libftl_hip.so (hand-written HIP kernels, C ABI in
+include/ftl_hip.h); this package is its host-side mirror of the reference's
+Python-visible classes.
+"""
+from .sampler import Sampler  # noqa: F401
+
+__all__ = ["Sampler"]
+__version__ = "0.1.0"

@@ -1,0 +1,96 @@
+"""ctypes binding of libftl_hip.so (declared in include/ftl_hip.h).
+
+The library is loaded from the package directory.  There is no fallback: when
+the library is missing or no HIP device is visible the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libftl_hip.so")
+
+FTL_OK, FTL_EINVAL, FTL_EHIP, FTL_ENOMEM, FTL_EUNSUPPORTED = 0, 1, 2, 3, 4
+FTL_FP64, FTL_FP32 = 0, 1
+
+_u32, _u64, _i32, _vp, _dp = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+
+# (name, argtypes, restype) of every exported entry point; the test suite
+# checks that include/ftl_hip.h declares exactly these.
+SIGNATURES = {
+    "ftl_last_error": ([], ctypes.c_char_p),
+    "ftl_version": ([], ctypes.c_char_p),
+    "ftl_device_count": ([], _i32),
+    "ftl_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
+    "ftl_sampler_create_device": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
+    "ftl_sampler_destroy": ([_vp], _i32),
+    "ftl_sampler_update": ([_vp, _u32, _vp], _i32),
+    "ftl_sampler_update_device": ([_vp, _u32, _vp], _i32),
+    "ftl_sampler_draw": ([_vp, _u32, _vp, _vp], _i32),
+    "ftl_sampler_draw_materialize": ([_vp, _u32, _vp, _vp, _vp, _vp], _i32),
+    "ftl_sampler_draw_device": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp], _i32),
+    "ftl_sampler_set_seed": ([_vp, _u64, _u64], _i32),
+    "ftl_sampler_get_tree": ([_vp, _u32, _vp, _u64], _i32),
+    "ftl_sampler_compute_m": ([_vp, _u32, _vp, _vp], _i32),
+    "ftl_sampler_info": ([_vp, _vp], _i32),
+    "ftl_sampler_stream": ([_vp], _vp),
+    "ftl_sampler_set_tuning": ([_vp, _u32, _u32], _i32),
+    "ftl_sampler_enable_timing": ([_vp, _i32], _i32),
+    "ftl_sampler_last_timing": ([_vp, _vp], _i32),
+}
+
+_lib = None
+
+
+class FTLError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m fast_tensor_leverage_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    try:
+        from . import _als_sigs
+        _als_sigs.bind(L)
+    except ImportError:
+        pass
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = load().ftl_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str = ""):
+    if rc == FTL_OK:
+        return
+    msg = f"{what}: {last_error()}" if what else last_error()
+    if rc == FTL_EINVAL:
+        raise ValueError(msg)
+    if rc == FTL_ENOMEM:
+        raise MemoryError(msg)
+    if rc == FTL_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise FTLError(msg)
+
+
+def device_count() -> int:
+    return int(load().ftl_device_count())
+
+
+def require_device():
+    if device_count() < 1:
+        raise FTLError("no HIP device visible: the MI355X path needs a GPU")

@@ -1,0 +1,580 @@
+// sampler.hip -- host side of the MI355X KRP leverage sampler: device memory,
+// tree builds, computeM and draw launches, and the C ABI of include/ftl_hip.h.
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ftl_common.hpp"
+#include "sampler_kernels.hpp"
+
+namespace ftl {
+
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string& msg) { g_err = msg; }
+const char* get_error() { return g_err.c_str(); }
+
+namespace {
+
+// Levels below this many from the bottom are stored FP32 in FP32 mode.
+constexpr uint32_t kDefaultFp32Levels = 6;
+
+struct ModeHost {
+    uint64_t I = 0, F = 0;
+    Shape sh{};
+    uint32_t d64 = 0;
+    uint64_t n64 = 0;
+    double* U64 = nullptr;
+    float* U32 = nullptr;
+    double* tree64 = nullptr;
+    float* tree32 = nullptr;
+};
+
+__global__ void k_convert_f32(const double* in, float* out, uint64_t n) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n;
+         w += (uint64_t)gridDim.x * blockDim.x)
+        out[w] = (float)in[w];
+}
+
+inline unsigned grid_for(uint64_t work, unsigned bs) {
+    uint64_t g = (work + bs - 1) / bs;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+struct Sampler {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0;
+    uint64_t J = 0, R = 0;
+    int precision = FTL_FP64;
+    uint32_t ts64 = 0, ts32 = 0;
+    uint32_t fp32_levels = kDefaultFp32Levels;
+    std::vector<ModeHost> modes;
+    double *pinv = nullptr, *W = nullptr, *E = nullptr, *etree = nullptr, *work = nullptr, *lam = nullptr;
+    uint32_t etree_nodes = 0;
+    // staging for the host-pointer API
+    uint64_t* samples_dev = nullptr;
+    double *h_dev = nullptr, *w_dev = nullptr, *draws_dev = nullptr;
+    uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
+    uint32_t sg_override = 0, grid_override = 0;
+    bool timing = false;
+    // events of every timed call since the last read-out: {start, after
+    // gram_pinv, after eigen_modes, after draw}
+    std::vector<std::array<hipEvent_t, 4>> pending;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double last_ms[3] = {0, 0, 0};
+    uint64_t last_count = 0;
+
+    hipEvent_t take_event() {
+        hipEvent_t e;
+        if (!event_pool.empty()) {
+            e = event_pool.back();
+            event_pool.pop_back();
+        } else {
+            FTL_HIP_CHECK(hipEventCreate(&e));
+        }
+        return e;
+    }
+
+    // Averages (ms) over the pending timed calls; blocks until they finished.
+    void collect_timing() {
+        double acc[3] = {0, 0, 0};
+        for (auto& q : pending) {
+            FTL_HIP_CHECK(hipEventSynchronize(q[3]));
+            for (int i = 0; i < 3; i++) {
+                float t = 0;
+                FTL_HIP_CHECK(hipEventElapsedTime(&t, q[i], q[i + 1]));
+                acc[i] += t;
+            }
+            for (auto e : q) event_pool.push_back(e);
+        }
+        last_count = pending.size();
+        for (int i = 0; i < 3; i++) last_ms[i] = last_count ? acc[i] / (double)last_count : 0.0;
+        pending.clear();
+    }
+    size_t bytes_held = 0;
+    std::vector<void*> allocs;
+
+    template <typename T>
+    T* dalloc(uint64_t count) {
+        void* p = nullptr;
+        const size_t bytes = (size_t)count * sizeof(T);
+        FTL_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+        allocs.push_back(p);
+        bytes_held += bytes;
+        return (T*)p;
+    }
+
+    void dfree(void* p, size_t bytes) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(p);
+        bytes_held -= bytes;
+    }
+
+    ~Sampler() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& q : pending)
+            for (auto e : q) (void)hipEventDestroy(e);
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void set_mode_precision(ModeHost& m) {
+        m.sh = make_shape(m.I, m.F);
+        const uint32_t depth = (uint32_t)m.sh.lfill_level + (m.sh.node_count > m.sh.nodes_upto_lfill ? 1 : 0);
+        if (precision == FTL_FP32) {
+            const uint32_t keep64 = depth + 1 > fp32_levels ? depth + 1 - fp32_levels : 0;
+            m.d64 = std::max<uint32_t>(1, keep64);
+        } else {
+            m.d64 = depth + 1;
+        }
+        const uint64_t full = m.d64 >= 63 ? ~0ull : ((1ull << m.d64) - 1);
+        m.n64 = std::min<uint64_t>(full, m.sh.node_count);
+    }
+
+    void build_mode(uint32_t k) {
+        ModeHost& m = modes[k];
+        BuildArgs ba;
+        ba.U = m.U64;
+        ba.tree = m.tree64;
+        ba.sh = m.sh;
+        ba.R = (uint32_t)R;
+        ba.ts64 = ts64;
+        ba.first_leaf = m.sh.node_count - m.sh.leaf_count;
+        const size_t lds = (size_t)m.F * R * sizeof(double);
+        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)m.sh.leaf_count), dim3(256), lds, stream, ba);
+        FTL_HIP_CHECK(hipGetLastError());
+        int64_t start = (int64_t)m.sh.nodes_before_lfill, end = (int64_t)ba.first_leaf;
+        for (int64_t level = (int64_t)m.sh.lfill_level; level >= 0; level--) {
+            if (end > start) {
+                const uint64_t work = (uint64_t)(end - start) * ts64;
+                hipLaunchKernelGGL(k_level_sum, dim3(grid_for(work, 256)), dim3(256), 0, stream, m.tree64,
+                                   (uint64_t)start, (uint64_t)end, ts64);
+                FTL_HIP_CHECK(hipGetLastError());
+            }
+            end = start;
+            start = ((start + 1) / 2) - 1;
+        }
+        if (precision == FTL_FP32) {
+            const uint64_t nel = m.I * R;
+            hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, stream, m.U64, m.U32, nel);
+            FTL_HIP_CHECK(hipGetLastError());
+            if (m.sh.node_count > m.n64) {
+                const uint64_t work = (m.sh.node_count - m.n64) * ts32;
+                hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(work, 256)), dim3(256), 0, stream, m.tree64,
+                                   m.tree32, m.n64, m.sh.node_count, ts64, ts32);
+                FTL_HIP_CHECK(hipGetLastError());
+            }
+        }
+    }
+
+    void init(uint32_t N_, const uint64_t* I, uint64_t J_, uint64_t R_, const double* const* U, bool on_device,
+              int prec) {
+        FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
+        FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
+        FTL_REQUIRE(J_ >= 1 && J_ <= (1ull << 40), "J must be in [1, 2^40]");
+        FTL_REQUIRE(prec == FTL_FP64 || prec == FTL_FP32, "precision must be FTL_FP64 or FTL_FP32");
+        FTL_REQUIRE(I != nullptr && U != nullptr, "null factor list");
+        N = N_;
+        J = J_;
+        R = R_;
+        precision = prec;
+        ts64 = (uint32_t)tri_stride64(R);
+        ts32 = (uint32_t)tri_stride32(R);
+        FTL_HIP_CHECK(hipGetDevice(&device));
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        static bool attrs_set = false;
+        if (!attrs_set) {
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_leaf_gram<256>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_gram_pinv<256>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_eigen_modes<256>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+            attrs_set = true;
+        }
+        modes.resize(N);
+        for (uint32_t k = 0; k < N; k++) {
+            FTL_REQUIRE(I[k] >= 1 && I[k] < (1ull << 40), "factor row counts must be in [1, 2^40)");
+            FTL_REQUIRE(U[k] != nullptr, "null factor matrix");
+            ModeHost& m = modes[k];
+            m.I = I[k];
+            m.F = std::min<uint64_t>(R, I[k]);
+            set_mode_precision(m);
+            m.U64 = dalloc<double>(m.I * R);
+            FTL_HIP_CHECK(hipMemcpyAsync(m.U64, U[k], m.I * R * sizeof(double),
+                                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+            m.tree64 = dalloc<double>(m.sh.node_count * ts64);
+            if (precision == FTL_FP32) {
+                m.U32 = dalloc<float>(m.I * R);
+                if (m.sh.node_count > m.n64) m.tree32 = dalloc<float>((m.sh.node_count - m.n64) * ts32);
+            }
+            build_mode(k);
+        }
+        const uint64_t R2 = R * R;
+        etree_nodes = (uint32_t)(2 * R - 1);
+        pinv = dalloc<double>(R2);
+        W = dalloc<double>(R2);
+        E = dalloc<double>(N * R2);
+        etree = dalloc<double>((uint64_t)N * etree_nodes * ts64);
+        work = dalloc<double>((uint64_t)(N + 1) * 2 * R2);
+        lam = dalloc<double>((uint64_t)(N + 1) * R);
+        FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R2 * sizeof(double), stream));
+        FTL_HIP_CHECK(hipMemsetAsync(etree, 0, (uint64_t)N * etree_nodes * ts64 * sizeof(double), stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void update(uint32_t j, const double* Uj, bool on_device) {
+        FTL_REQUIRE(j < N, "update_sampler: mode index out of range");
+        FTL_REQUIRE(Uj != nullptr, "update_sampler: null factor");
+        ModeHost& m = modes[j];
+        FTL_HIP_CHECK(hipMemcpyAsync(m.U64, Uj, m.I * R * sizeof(double),
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+        build_mode(j);
+    }
+
+    std::vector<int> sampled_modes(uint32_t j) const {
+        std::vector<int> out;
+        for (uint32_t k = 0; k < N; k++)
+            if (j >= N || k != j) out.push_back((int)k);
+        return out;
+    }
+
+    ComputeMArgs computeM_args(const std::vector<int>& order) {
+        ComputeMArgs a{};
+        for (uint32_t k = 0; k < N; k++) a.roots[k] = modes[k].tree64;
+        for (size_t t = 0; t < order.size(); t++) a.modes[t] = order[t];
+        a.m = (int)order.size();
+        a.R = (int)R;
+        a.pinv = pinv;
+        a.W = W;
+        a.E = E;
+        a.etree = etree;
+        a.work = work;
+        a.lam_out = lam;
+        a.ts64 = ts64;
+        a.etree_nodes = etree_nodes;
+        return a;
+    }
+
+    void compute_m(uint32_t j, hipStream_t st, hipEvent_t* evs = nullptr) {
+        std::vector<int> order = sampled_modes(j);
+        FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
+        ComputeMArgs a = computeM_args(order);
+        const size_t lds = 2 * R * R <= 8192 ? 2 * R * R * sizeof(double) : 0;
+        if (evs) FTL_HIP_CHECK(hipEventRecord(evs[0], st));
+        hipLaunchKernelGGL(k_gram_pinv<256>, dim3(1), dim3(256), lds, st, a);
+        FTL_HIP_CHECK(hipGetLastError());
+        if (evs) FTL_HIP_CHECK(hipEventRecord(evs[1], st));
+        hipLaunchKernelGGL(k_eigen_modes<256>, dim3(a.m), dim3(256), lds, st, a);
+        FTL_HIP_CHECK(hipGetLastError());
+        if (evs) FTL_HIP_CHECK(hipEventRecord(evs[2], st));
+    }
+
+    template <int RMAX, int SG, bool USE32>
+    void launch_draw(const DrawArgs& a, hipStream_t st) {
+        constexpr int BS = 256;
+        const uint64_t per_block = BS / SG;
+        uint64_t blocks = (J + per_block - 1) / per_block;
+        if (grid_override) blocks = std::min<uint64_t>(blocks, grid_override);
+        blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 20));
+        hipLaunchKernelGGL((k_krp_draw<RMAX, SG, BS, USE32>), dim3((unsigned)blocks), dim3(BS), 0, st, a);
+        FTL_HIP_CHECK(hipGetLastError());
+    }
+
+    template <int RMAX, int SG>
+    void launch_draw_p(const DrawArgs& a, hipStream_t st) {
+        if (precision == FTL_FP32)
+            launch_draw<RMAX, SG, true>(a, st);
+        else
+            launch_draw<RMAX, SG, false>(a, st);
+    }
+
+    void dispatch_draw(const DrawArgs& a, hipStream_t st) {
+        const uint32_t sg = sg_override;
+        if (R <= 8) {
+            launch_draw_p<8, 8>(a, st);
+        } else if (R <= 16) {
+            launch_draw_p<16, 16>(a, st);
+        } else if (R <= 32) {
+            // measured at N=3, I=2^23, R=32, J=5e4: SG=32 fastest for FP64,
+            // SG=64 for FP32
+            const uint32_t s = sg ? sg : (precision == FTL_FP32 ? 64u : 32u);
+            if (s == 8) launch_draw_p<32, 8>(a, st);
+            else if (s == 16) launch_draw_p<32, 16>(a, st);
+            else if (s == 64) launch_draw_p<32, 64>(a, st);
+            else launch_draw_p<32, 32>(a, st);
+        } else if (R <= 64) {
+            if (sg == 32) launch_draw_p<64, 32>(a, st);
+            else launch_draw_p<64, 64>(a, st);
+        } else {
+            launch_draw_p<128, 64>(a, st);
+        }
+    }
+
+    void draw_device(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws, hipStream_t st) {
+        FTL_REQUIRE(samples != nullptr, "null samples buffer");
+        if (!st) st = stream;
+        std::vector<int> order = sampled_modes(j);
+        std::array<hipEvent_t, 4> evs{};
+        if (timing)
+            for (auto& e : evs) e = take_event();
+        compute_m(j, st, timing ? evs.data() : nullptr);
+        DrawArgs a{};
+        for (uint32_t k = 0; k < N; k++) {
+            const ModeHost& m = modes[k];
+            ModeDev& d = a.mode[k];
+            d.tree64 = m.tree64;
+            d.tree32 = m.tree32;
+            d.U64 = m.U64;
+            d.U32 = m.U32;
+            d.etree = etree + (uint64_t)k * etree_nodes * ts64;
+            d.E = E + (uint64_t)k * R * R;
+            d.sh = m.sh;
+            d.n64 = m.n64;
+            d.d64 = m.d64;
+        }
+        for (size_t t = 0; t < order.size(); t++) a.order[t] = order[t];
+        a.m = (int)order.size();
+        a.N = (int)N;
+        a.J = J;
+        a.R = (int)R;
+        a.ts64 = ts64;
+        a.ts32 = ts32;
+        a.esh = make_shape(R, 1);
+        a.pinv = pinv;
+        a.draws = draws;
+        a.seed = seed;
+        a.call = call;
+        a.samples = samples;
+        a.h_out = h;
+        a.w_out = w;
+        dispatch_draw(a, st);
+        if (timing) {
+            FTL_HIP_CHECK(hipEventRecord(evs[3], st));
+            pending.push_back(evs);
+        }
+        if (!draws) call++;
+    }
+
+    void ensure_staging() {
+        if (!samples_dev) {
+            samples_dev = dalloc<uint64_t>(N * J);
+            h_dev = dalloc<double>(J * R);
+            w_dev = dalloc<double>(J);
+            draws_dev = dalloc<double>(2 * N * J);
+        }
+    }
+
+    void draw_host(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws) {
+        FTL_REQUIRE(samples != nullptr, "null samples array");
+        ensure_staging();
+        if (draws)
+            FTL_HIP_CHECK(hipMemcpyAsync(draws_dev, draws, 2 * N * J * sizeof(double), hipMemcpyHostToDevice,
+                                         stream));
+        draw_device(j, samples_dev, h ? h_dev : nullptr, w ? w_dev : nullptr, draws ? draws_dev : nullptr,
+                    stream);
+        for (int k : sampled_modes(j))
+            FTL_HIP_CHECK(hipMemcpyAsync(samples + (uint64_t)k * J, samples_dev + (uint64_t)k * J,
+                                         J * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        if (h) FTL_HIP_CHECK(hipMemcpyAsync(h, h_dev, J * R * sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (w) FTL_HIP_CHECK(hipMemcpyAsync(w, w_dev, J * sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void get_tree(uint32_t k, double* out, uint64_t capacity) {
+        FTL_REQUIRE(k < N, "mode index out of range");
+        const ModeHost& m = modes[k];
+        FTL_REQUIRE(capacity >= m.sh.node_count, "output holds fewer nodes than the tree");
+        std::vector<double> packed(m.sh.node_count * ts64);
+        FTL_HIP_CHECK(hipMemcpyAsync(packed.data(), m.tree64, packed.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        const uint64_t R2 = R * R;
+        for (uint64_t c = 0; c < m.sh.node_count; c++) {
+            double* dst = out + c * R2;
+            std::fill(dst, dst + R2, 0.0);
+            for (uint64_t a = 0; a < R; a++)
+                for (uint64_t b = a; b < R; b++) dst[a * R + b] = packed[c * ts64 + tri_offset(R, a, b)];
+        }
+    }
+
+    void compute_m_host(uint32_t j, double* pinv_out, double* E_out) {
+        FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R * R * sizeof(double), stream));
+        compute_m(j, stream);
+        if (pinv_out)
+            FTL_HIP_CHECK(hipMemcpyAsync(pinv_out, pinv, R * R * sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (E_out)
+            FTL_HIP_CHECK(hipMemcpyAsync(E_out, E, N * R * R * sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+};
+
+}  // namespace ftl
+
+struct ftl_sampler {
+    ftl::Sampler impl;
+};
+
+using ftl::guarded;
+
+extern "C" {
+
+const char* ftl_last_error(void) { return ftl::get_error(); }
+
+const char* ftl_version(void) { return "ftl_hip 0.1 (gfx950)"; }
+
+int ftl_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int create_common(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U,
+                         bool on_device, int precision, ftl_sampler** out) {
+    return guarded([&] {
+        FTL_REQUIRE(out != nullptr, "null output handle");
+        *out = nullptr;
+        int ndev = 0;
+        FTL_HIP_CHECK(hipGetDeviceCount(&ndev));
+        FTL_REQUIRE(ndev > 0, "no HIP device visible");
+        std::unique_ptr<ftl_sampler> s(new ftl_sampler());
+        s->impl.init(N, I, J, R, U, on_device, precision);
+        *out = s.release();
+    });
+}
+
+int ftl_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
+                       int precision, ftl_sampler** out) {
+    return create_common(N, I, J, R, U_host, false, precision, out);
+}
+
+int ftl_sampler_create_device(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_dev,
+                              int precision, ftl_sampler** out) {
+    return create_common(N, I, J, R, U_dev, true, precision, out);
+}
+
+int ftl_sampler_destroy(ftl_sampler* s) {
+    return guarded([&] { delete s; });
+}
+
+int ftl_sampler_update(ftl_sampler* s, uint32_t j, const double* Uj_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.update(j, Uj_host, false);
+        FTL_HIP_CHECK(hipStreamSynchronize(s->impl.stream));
+    });
+}
+
+int ftl_sampler_update_device(ftl_sampler* s, uint32_t j, const double* Uj_dev) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.update(j, Uj_dev, true);
+    });
+}
+
+int ftl_sampler_draw(ftl_sampler* s, uint32_t j, uint64_t* samples_host, const double* draws_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.draw_host(j, samples_host, nullptr, nullptr, draws_host);
+    });
+}
+
+int ftl_sampler_draw_materialize(ftl_sampler* s, uint32_t j, uint64_t* samples_host, double* h_host,
+                                 double* weights_host, const double* draws_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        FTL_REQUIRE(h_host && weights_host, "null h or weights array");
+        s->impl.draw_host(j, samples_host, h_host, weights_host, draws_host);
+    });
+}
+
+int ftl_sampler_draw_device(ftl_sampler* s, uint32_t j, uint64_t* samples_dev, double* h_dev, double* weights_dev,
+                            const double* draws_dev, void* stream) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.draw_device(j, samples_dev, h_dev, weights_dev, draws_dev, (hipStream_t)stream);
+    });
+}
+
+int ftl_sampler_set_seed(ftl_sampler* s, uint64_t seed, uint64_t counter) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.seed = seed;
+        s->impl.call = counter;
+    });
+}
+
+int ftl_sampler_get_tree(ftl_sampler* s, uint32_t k, double* G_host, uint64_t node_capacity) {
+    return guarded([&] {
+        FTL_REQUIRE(s && G_host, "null argument");
+        s->impl.get_tree(k, G_host, node_capacity);
+    });
+}
+
+int ftl_sampler_compute_m(ftl_sampler* s, uint32_t j, double* pinv_host, double* E_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.compute_m_host(j, pinv_host, E_host);
+    });
+}
+
+int ftl_sampler_info(ftl_sampler* s, uint64_t* out) {
+    return guarded([&] {
+        FTL_REQUIRE(s && out, "null argument");
+        const auto& m = s->impl;
+        out[0] = m.N;
+        out[1] = m.J;
+        out[2] = m.R;
+        out[3] = (uint64_t)m.precision;
+        out[4] = m.modes[0].d64;
+        out[5] = m.bytes_held;
+        out[6] = m.modes[0].sh.node_count;
+        out[7] = m.modes[0].sh.lfill_level;
+    });
+}
+
+void* ftl_sampler_stream(ftl_sampler* s) { return s ? (void*)s->impl.stream : nullptr; }
+
+int ftl_sampler_set_tuning(ftl_sampler* s, uint32_t group_lanes, uint32_t grid_blocks) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        FTL_REQUIRE(group_lanes == 0 || group_lanes == 8 || group_lanes == 16 || group_lanes == 32 ||
+                        group_lanes == 64,
+                    "group_lanes must be 0, 8, 16, 32 or 64");
+        s->impl.sg_override = group_lanes;
+        s->impl.grid_override = grid_blocks;
+    });
+}
+
+int ftl_sampler_enable_timing(ftl_sampler* s, int enable) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null sampler");
+        s->impl.timing = enable != 0;
+    });
+}
+
+int ftl_sampler_last_timing(ftl_sampler* s, double* out4) {
+    return guarded([&] {
+        FTL_REQUIRE(s && out4, "null argument");
+        s->impl.collect_timing();
+        for (int i = 0; i < 3; i++) out4[i] = s->impl.last_ms[i];
+        out4[3] = (double)s->impl.last_count;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,719 @@
+// sampler_kernels.hpp -- gfx950 kernels of the exact KRP leverage sampler.
+//
+//   k_leaf_gram     leaf Grams U_blk^T U_blk (partition_tree.hpp:136-154)
+//   k_level_sum     parent = left + right, one tree level (partition_tree.hpp:156-168)
+//   k_tree_to_f32   FP32 copy of the bottom levels of an FP64 tree
+//   k_gram_pinv     computeM part 1: G = Hadamard of sampled root Grams, its
+//                   eigendecomposition and pinv (efficient_krp_sampler.hpp:90-150)
+//   k_eigen_modes   computeM part 2: per-mode eigen matrices E_k and eigen
+//                   trees (efficient_krp_sampler.hpp:152-193)
+//   k_krp_draw      the whole KRPDrawSamples after computeM, fused: for every
+//                   sample, every sampled mode's eigen descent, Gram descent,
+//                   leaf row scan and h update, then the weight
+//                   (efficient_krp_sampler.hpp:209-268, partition_tree.hpp:232-405)
+//
+// Trees are stored level-major (heap order) with each node's R x R Gram as a
+// packed upper triangle.  In FP32 mode the top `d64` levels stay FP64 and the
+// levels below are FP32: a descent compares u against a running cutoff whose
+// absolute error is set by the top levels, so those must be FP64, while the
+// bottom levels (where the HBM traffic is) tolerate FP32.
+#pragma once
+
+#include "ftl_common.hpp"
+
+namespace ftl {
+
+constexpr int kMaxModes = 16;
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ inline void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP lane permutation of a double (two 32-bit halves).
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over a group of SG consecutive lanes.  Each step adds a lane's value to
+// that of a partner lane such that both end with the same operands, so every
+// lane of the group ends with the bitwise identical sum (IEEE addition is
+// commutative).  Within a 16-lane row the partners come from DPP (quad
+// permutations, half-row and row mirrors); across rows from ds_bpermute.
+template <int SG>
+__device__ inline double group_sum(double v) {
+    static_assert(SG >= 2 && SG <= 64 && (SG & (SG - 1)) == 0, "group width");
+    v += dpp_f64<0xB1>(v);                     // quad_perm [1,0,3,2]
+    if constexpr (SG >= 4) v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    if constexpr (SG >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror
+    if constexpr (SG >= 16) v += dpp_f64<0x140>(v); // row_mirror
+    if constexpr (SG >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (SG >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+template <int SG>
+__device__ inline double group_incl_scan(double v, int g) {
+#pragma unroll
+    for (int o = 1; o < SG; o <<= 1) {
+        double y = __shfl_up(v, o, SG);
+        if (g >= o) v += y;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Tree build kernels (FP64).  One block per leaf: the leaf's row block is
+// staged in LDS, each thread produces packed upper-triangle entries as
+// sequential sums over the rows (no contraction, so the result is the exactly
+// rounded sequential sum the reference's dsyrk restatement produces).
+// ---------------------------------------------------------------------------
+struct BuildArgs {
+    const double* U;   // n x R
+    double* tree;      // node_count x ts64
+    Shape sh;
+    uint32_t R;
+    uint32_t ts64;
+    uint64_t first_leaf;  // node_count - leaf_count
+};
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double lds_u[];
+    const uint64_t i = blockIdx.x;  // leaf position
+    const uint64_t node = a.first_leaf + i;
+    const uint64_t idx = leaf_index(a.sh, node);
+    const uint32_t R = a.R;
+    const uint64_t r0 = idx * a.sh.F;
+    const uint32_t rows = (uint32_t)min<uint64_t>(a.sh.F, a.sh.n - r0);
+    for (uint32_t e = threadIdx.x; e < rows * R; e += BS) lds_u[e] = a.U[r0 * R + e];
+    __syncthreads();
+    const uint32_t T = (uint32_t)tri_size(R);
+    double* out = a.tree + node * (uint64_t)a.ts64;
+    for (uint32_t e = threadIdx.x; e < a.ts64; e += BS) {
+        double acc = 0.0;
+        if (e < T) {
+            // invert e -> (row a, col b)
+            uint32_t ra = 0, off = e;
+            while (off >= R - ra) {
+                off -= R - ra;
+                ra++;
+            }
+            const uint32_t cb = ra + off;
+            for (uint32_t t = 0; t < rows; t++)
+                acc = acc + lds_u[t * R + ra] * lds_u[t * R + cb];
+        }
+        out[e] = acc;
+    }
+}
+
+__global__ void k_level_sum(double* tree, uint64_t start, uint64_t end, uint32_t ts64) {
+    const uint64_t total = (end - start) * ts64;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = start + w / ts64, e = w % ts64;
+        tree[c * ts64 + e] = tree[(2 * c + 1) * ts64 + e] + tree[(2 * c + 2) * ts64 + e];
+    }
+}
+
+__global__ void k_tree_to_f32(const double* tree64, float* tree32, uint64_t n64, uint64_t node_count,
+                              uint32_t ts64, uint32_t ts32) {
+    const uint64_t total = (node_count - n64) * ts32;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = n64 + w / ts32, e = w % ts32;
+        tree32[w] = e < ts64 ? (float)tree64[c * ts64 + e] : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi eigensolver for one symmetric R x R matrix held by a block
+// (A and V in LDS or global scratch).  Round-robin pair order; within a round
+// all rotations are computed from the same A, then all row updates, then all
+// column updates (A and V).  Rotation annihilating A[p][q] (Golub & Van Loan
+// Alg. 8.5.2), skipped when |A[p][q]| <= eps*sqrt(|A[p][p] A[q][q]|).
+// Products and sums are rounded individually (no contraction).  On return lam
+// is ascending and Vout column v holds the eigenvector of lam[v].
+// ---------------------------------------------------------------------------
+struct JacobiShared {
+    int p[64], q[64], act[64];
+    double c[64], s[64];
+};
+
+template <int BS>
+__device__ void jacobi_block(double* A, double* V, int R, double* lam, double* Vout,
+                             JacobiShared& js, double* diag_tmp) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int n2 = R + (R & 1);
+    const int npairs = n2 / 2;
+    const double eps = 2.220446049250313e-16;
+    for (int e = tid; e < R * R; e += BS) V[e] = (e / R == e % R) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 40; sweep++) {
+        int any = 0;
+        for (int round = 0; round + 1 < n2; round++) {
+            if (tid < npairs) {
+                const int k = tid;
+                int p = k == 0 ? 0 : 1 + ((k - 1 + round) % (n2 - 1));
+                const int kk = n2 - 1 - k;
+                int q = kk == 0 ? 0 : 1 + ((kk - 1 + round) % (n2 - 1));
+                if (p > q) {
+                    int t = p;
+                    p = q;
+                    q = t;
+                }
+                int act = 0;
+                double c = 1.0, s = 0.0;
+                if (q < R) {
+                    const double apq = A[p * R + q], app = A[p * R + p], aqq = A[q * R + q];
+                    const double thr = eps * sqrt(fabs(app * aqq));
+                    if (!(fabs(apq) <= thr || apq == 0.0)) {
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        double t;
+                        if (fabs(tau) > 1e150) {
+                            t = 0.5 / tau;
+                        } else {
+                            const double sq = sqrt(1.0 + tau * tau);
+                            t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sq);
+                        }
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                        act = 1;
+                    }
+                }
+                js.p[k] = p;
+                js.q[k] = q;
+                js.c[k] = c;
+                js.s[k] = s;
+                js.act[k] = act;
+                any |= act;
+            }
+            __syncthreads();
+            for (int it = tid; it < npairs * R; it += BS) {
+                const int k = it / R, j = it % R;
+                if (!js.act[k]) continue;
+                const int p = js.p[k], q = js.q[k];
+                const double c = js.c[k], s = js.s[k];
+                const double ap = A[p * R + j], aq = A[q * R + j];
+                A[p * R + j] = c * ap - s * aq;
+                A[q * R + j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            for (int it = tid; it < npairs * R; it += BS) {
+                const int k = it / R, i = it % R;
+                if (!js.act[k]) continue;
+                const int p = js.p[k], q = js.q[k];
+                const double c = js.c[k], s = js.s[k];
+                const double ap = A[i * R + p], aq = A[i * R + q];
+                A[i * R + p] = c * ap - s * aq;
+                A[i * R + q] = s * ap + c * aq;
+                const double vp = V[i * R + p], vq = V[i * R + q];
+                V[i * R + p] = c * vp - s * vq;
+                V[i * R + q] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+        if (!__syncthreads_or(any)) break;
+    }
+    // stable ascending sort of the diagonal
+    for (int v = tid; v < R; v += BS) diag_tmp[v] = A[v * R + v];
+    __syncthreads();
+    for (int v = tid; v < R; v += BS) {
+        const double d = diag_tmp[v];
+        int rank = 0;
+        for (int w = 0; w < R; w++) {
+            const double dw = diag_tmp[w];
+            rank += (dw < d) || (dw == d && w < v);
+        }
+        lam[rank] = d;
+        for (int i = 0; i < R; i++) Vout[i * R + rank] = V[i * R + v];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// computeM kernels
+// ---------------------------------------------------------------------------
+struct ComputeMArgs {
+    const double* roots[kMaxModes];  // packed FP64 root Gram of each mode (node 0)
+    int modes[kMaxModes];            // sampled modes in increasing order
+    int m;                           // number of sampled modes
+    int R;
+    double* pinv;    // R x R full symmetric
+    double* W;       // R x R, columns = eigvecs / sqrt(lambda), descending lambda
+    double* E;       // N x R x R
+    double* etree;   // N x etree_nodes x ts64
+    double* work;    // per block: 2 * R * R doubles (A, V) when they do not fit LDS
+    double* lam_out; // (m+1) x R eigenvalues (diagnostics)
+    uint32_t ts64;
+    uint32_t etree_nodes;
+};
+
+__device__ inline double packed_get(const double* P, int R, int a, int b) {
+    return a <= b ? P[tri_offset(R, a, b)] : P[tri_offset(R, b, a)];
+}
+
+// Block-level A/V storage: LDS when 2 R^2 doubles fit, global scratch otherwise.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double diag_tmp[128];
+    const int R = a.R, R2 = R * R;
+    const bool in_lds = 2 * R2 <= 8192;
+    double* A = in_lds ? smem : a.work;
+    double* V = in_lds ? smem + R2 : a.work + R2;
+    double* lam = a.lam_out;
+    double* Vs = a.pinv;  // sorted eigenvectors parked in the pinv buffer first
+    for (int e = threadIdx.x; e < R2; e += BS) {
+        const int r = e / R, c = e % R;
+        double g = 1.0;
+        // suffix order of the reference: G_{m-1} (.) ... (.) G_0 applied
+        // from the last sampled mode down (efficient_krp_sampler.hpp:95-105)
+        for (int t = a.m - 1; t >= 0; t--) g = packed_get(a.roots[a.modes[t]], R, r, c) * g;
+        A[e] = g;
+    }
+    __syncthreads();
+    jacobi_block<BS>(A, V, R, lam, Vs, js, diag_tmp);
+    // W[u][R-1-v] = V[u][v] / sqrt(lam[v]) when lam[v] > 1e-8 (126-138)
+    for (int e = threadIdx.x; e < R2; e += BS) {
+        const int u = e / R, v = e % R;
+        const double l = lam[v];
+        a.W[u * R + (R - 1 - v)] = l > 1e-8 ? Vs[u * R + v] / sqrt(l) : 0.0;
+    }
+    __syncthreads();
+    // pinv = W W^T (140-150), symmetric storage
+    for (int e = threadIdx.x; e < R2; e += BS) {
+        const int r = e / R, c = e % R;
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        double acc = 0.0;
+        for (int k = 0; k < R; k++) acc = acc + a.W[lo * R + k] * a.W[hi * R + k];
+        A[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < R2; e += BS) a.pinv[e] = A[e];
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_eigen_modes(ComputeMArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double diag_tmp[128];
+    const int t = blockIdx.x;  // position in the sampled-mode list
+    const int k = a.modes[t];
+    const int R = a.R, R2 = R * R;
+    double* E = a.E + (uint64_t)k * R2;
+    if (t + 1 < a.m) {
+        const bool in_lds = 2 * R2 <= 8192;
+        double* A = in_lds ? smem : a.work + (uint64_t)(t + 1) * 2 * R2;
+        double* V = A + R2;
+        double* lam = a.lam_out + (uint64_t)(t + 1) * R;
+        // P_k = (G_{modes[t+1]} (.) (... (.) G_{modes[m-1]})) (.) pinv
+        for (int e = threadIdx.x; e < R2; e += BS) {
+            const int r = e / R, c = e % R;
+            double g = 1.0;
+            for (int s = a.m - 1; s > t; s--) g = packed_get(a.roots[a.modes[s]], R, r, c) * g;
+            A[e] = g * a.pinv[e];
+        }
+        __syncthreads();
+        // sorted eigenvectors land in A (only its diagonal is still needed
+        // by the sort, and that is copied out before the permutation)
+        jacobi_block<BS>(A, V, R, lam, A, js, diag_tmp);
+        for (int e = threadIdx.x; e < R2; e += BS) {
+            const int r = e / R, u = e % R;
+            // sqrt(mu) clamped at 0 (the reference takes sqrt of a negative
+            // eigenvalue and produces NaN, efficient_krp_sampler.hpp:176-180)
+            const double mu = lam[r];
+            E[r * R + u] = A[u * R + r] * sqrt(mu > 0.0 ? mu : 0.0);
+        }
+    } else {
+        for (int e = threadIdx.x; e < R2; e += BS) {
+            const int r = e / R, u = e % R;
+            E[r * R + u] = a.W[u * R + r];
+        }
+    }
+    __syncthreads();
+    // Eigen tree over the R rows of E with F = 1, then every node (.) G_k
+    // (187-193).  Leaves: node L-1+i holds row leaf_index(node).
+    const Shape es = make_shape(R, 1);
+    const int T = (int)tri_size(R);
+    double* tree = a.etree + (uint64_t)k * a.etree_nodes * a.ts64;
+    const uint64_t first_leaf = es.node_count - es.leaf_count;
+    for (uint64_t w = threadIdx.x; w < es.leaf_count * a.ts64; w += BS) {
+        const uint64_t i = w / a.ts64, e = w % a.ts64;
+        const uint64_t node = first_leaf + i;
+        const uint64_t row = leaf_index(es, node);
+        double v = 0.0;
+        if ((int)e < T) {
+            uint32_t ra = 0, off = (uint32_t)e;
+            while (off >= (uint32_t)R - ra) {
+                off -= R - ra;
+                ra++;
+            }
+            v = E[row * R + ra] * E[row * R + ra + off];
+        }
+        tree[node * a.ts64 + e] = v;
+    }
+    __syncthreads();
+    int64_t start = (int64_t)es.nodes_before_lfill, end = (int64_t)first_leaf;
+    for (int64_t level = (int64_t)es.lfill_level; level >= 0; level--) {
+        for (uint64_t w = threadIdx.x; w < (uint64_t)(end - start) * a.ts64; w += BS) {
+            const uint64_t c = start + w / a.ts64, e = w % a.ts64;
+            tree[c * a.ts64 + e] = tree[(2 * c + 1) * a.ts64 + e] + tree[(2 * c + 2) * a.ts64 + e];
+        }
+        __syncthreads();
+        end = start;
+        start = ((start + 1) / 2) - 1;
+    }
+    const double* Gk = a.roots[k];
+    for (uint64_t w = threadIdx.x; w < es.node_count * a.ts64; w += BS) {
+        const uint64_t e = w % a.ts64;
+        if ((int)e < T) tree[w] = tree[w] * Gk[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused draw kernel
+// ---------------------------------------------------------------------------
+struct ModeDev {
+    const double* tree64;  // nodes [0, n64), stride ts64
+    const float* tree32;   // nodes [n64, node_count) at (c - n64) * ts32
+    const double* U64;     // I x R
+    const float* U32;      // I x R, FP32 mode only
+    const double* etree;   // eigen tree, stride ts64
+    const double* E;       // R x R
+    Shape sh;
+    uint64_t n64;          // first FP32 node (= node_count in FP64 mode)
+    uint32_t d64;          // levels [0, d64) are FP64
+};
+
+struct DrawArgs {
+    ModeDev mode[kMaxModes];
+    int order[kMaxModes];  // sampled modes, increasing
+    int m;                 // number of sampled modes
+    int N;
+    uint64_t J;
+    int R;
+    uint32_t ts64, ts32;
+    Shape esh;             // eigen tree shape (R, 1)
+    const double* pinv;    // R x R
+    const double* draws;   // 2*N*J or null
+    uint64_t seed, call;
+    uint64_t* samples;     // N x J
+    double* h_out;         // J x R or null
+    double* w_out;         // J or null
+};
+
+template <int RMAX, int SG>
+struct DrawCfg {
+    static constexpr int TMAX = RMAX * (RMAX + 1) / 2;
+    static constexpr int EPL = (TMAX + SG - 1) / SG;   // packed entries per lane
+    static constexpr int HPL = (RMAX + SG - 1) / SG;   // vector entries per lane
+    static constexpr int RPL = (RMAX + SG - 1) / SG;   // leaf rows per lane
+};
+
+// Coefficients of x^T G x over the packed entries this lane owns:
+// c = x_a x_b (diagonal) or 2 x_a x_b (off-diagonal), entry e = g + SG*i.
+template <int RMAX, int SG>
+__device__ inline void make_coefs(const double* xs, int R, int g, int epl,
+                                  double (&c)[DrawCfg<RMAX, SG>::EPL]) {
+    int a = 0, off = g;
+    while (a < R && off >= R - a) {
+        off -= R - a;
+        a++;
+    }
+#pragma unroll
+    for (int i = 0; i < DrawCfg<RMAX, SG>::EPL; i++) {
+        double v = 0.0;
+        if (i < epl && a < R) {
+            const double p = xs[a] * xs[a + off];
+            v = off == 0 ? p : 2.0 * p;
+        }
+        c[i] = v;
+        off += SG;
+        while (a < R && off >= R - a) {
+            off -= R - a;
+            a++;
+        }
+    }
+}
+
+// x^T G x over one packed node.  The loads are unconditional so that all EPL
+// of them are in flight together: an entry index past the packed size T is
+// clamped to T-1 (inside the node) and its coefficient is zero.  (The node
+// stride is only rounded to 16/32 entries, which SG * ceil(T/SG) can exceed.)
+template <typename TN, typename TC, int EPL, int SG>
+__device__ inline double node_qform(const TN* __restrict__ node, const TC (&c)[EPL], int g, int epl, int T) {
+    TN v[EPL];
+#pragma unroll
+    for (int i = 0; i < EPL; i++) {
+        const int e = g + SG * i;
+        v[i] = node[e < T ? e : T - 1];
+    }
+    TC acc0 = 0, acc1 = 0;
+#pragma unroll
+    for (int i = 0; i < EPL; i += 2) {
+        acc0 += c[i] * (TC)v[i];
+        if (i + 1 < EPL) acc1 += c[i + 1] * (TC)v[i + 1];
+    }
+    (void)epl;
+    return group_sum<SG>((double)acc0 + (double)acc1);
+}
+
+// One inverse-CDF descent (partition_tree.hpp:251-332) for the group's sample.
+template <int RMAX, int SG>
+__device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
+                                   const double (&c64)[DrawCfg<RMAX, SG>::EPL],
+                                   const float (&c32)[DrawCfg<RMAX, SG>::EPL], bool use32,
+                                   const double* tree64, double u, int g, int epl, int T, double& low,
+                                   double& high) {
+    constexpr int EPL = DrawCfg<RMAX, SG>::EPL;
+    const double m = node_qform<double, double, EPL, SG>(tree64, c64, g, epl, T);
+    uint64_t c = 0;
+    low = 0.0;
+    high = 1.0;
+    for (uint32_t d = 0; d < sh.lfill_level; d++) {
+        const uint64_t child = 2 * c + 1;
+        double mL;
+        if (!use32 || d + 1 < md.d64)
+            mL = node_qform<double, double, EPL, SG>(tree64 + child * ts64, c64, g, epl, T);
+        else
+            mL = node_qform<float, float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c32, g, epl, T);
+        const double cutoff = low + mL / m;
+        if (u <= cutoff) {
+            c = child;
+            high = cutoff;
+        } else {
+            c = child + 1;
+            low = cutoff;
+        }
+    }
+    if (sh.node_count > sh.nodes_before_lfill) {
+        const bool leaf = 2 * c + 1 >= sh.node_count;
+        if (!leaf) {
+            const uint64_t child = 2 * c + 1;
+            double mL;
+            if (!use32 || sh.lfill_level + 1 < md.d64)
+                mL = node_qform<double, double, EPL, SG>(tree64 + child * ts64, c64, g, epl, T);
+            else
+                mL = node_qform<float, float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c32, g, epl, T);
+            const double cutoff = low + mL / m;
+            if (u <= cutoff) {
+                c = child;
+                high = cutoff;
+            } else {
+                c = child + 1;
+                low = cutoff;
+            }
+        }
+    }
+    return c;
+}
+
+// U_row . x for one leaf row.  With R == RMAX the row is read with 16-byte
+// vector loads (rows are 16-byte aligned then); otherwise entry by entry with
+// the column index clamped (x is zero past R).  All loads are unconditional.
+template <typename TU, int RMAX>
+__device__ inline TU row_dot(const TU* __restrict__ row, const double* xs, int R) {
+    constexpr int W = 16 / sizeof(TU);
+    typedef TU V __attribute__((ext_vector_type(W)));
+    TU acc0 = 0, acc1 = 0;
+    if (R == RMAX) {
+        const V* rv = reinterpret_cast<const V*>(row);
+        V buf[RMAX / W];
+#pragma unroll
+        for (int q = 0; q < RMAX / W; q++) buf[q] = rv[q];
+#pragma unroll
+        for (int q = 0; q < RMAX / W; q++)
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                if (w & 1) acc1 += buf[q][w] * (TU)xs[q * W + w];
+                else acc0 += buf[q][w] * (TU)xs[q * W + w];
+            }
+    } else {
+        TU buf[RMAX];
+#pragma unroll
+        for (int b = 0; b < RMAX; b++) buf[b] = row[b < R ? b : R - 1];
+#pragma unroll
+        for (int b = 0; b < RMAX; b++) {
+            if (b & 1) acc1 += buf[b] * (TU)xs[b];
+            else acc0 += buf[b] * (TU)xs[b];
+        }
+    }
+    return acc0 + acc1;
+}
+
+// Leaf row scan (partition_tree.hpp:336-391): q_t = (U_t . x)^2 for the rows
+// of the leaf, inclusive prefix over the group, P_{t+1} = prefix_t / total,
+// result = first t <= F-2 with frac < P_{t+1}, else F-1 (clamped to the last
+// real row of a partial leaf).  Lane g owns rows g, g+SG, ...
+template <typename TU, int RMAX, int SG>
+__device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint32_t rows, uint32_t F,
+                                     int R, const double* xs, double frac, int g, int lane) {
+    constexpr int RPL = DrawCfg<RMAX, SG>::RPL;
+    double q2[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        const uint32_t t = g + SG * k;
+        const uint32_t tc = t < rows ? t : rows - 1;
+        const TU d = row_dot<TU, RMAX>(U + (r0 + tc) * (uint64_t)R, xs, R);
+        q2[k] = t < rows ? (double)d * (double)d : 0.0;
+    }
+    double incl[RPL];
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        if (SG * k < (int)F) {
+            const double v = group_incl_scan<SG>(q2[k], g) + run;
+            incl[k] = v;
+            run = __shfl(v, SG - 1, SG);
+        } else {
+            incl[k] = run;
+        }
+    }
+    const double total = run;
+    uint32_t res = F - 1;
+    bool found = false;
+    const int gbase = lane - g;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        const uint32_t t = g + SG * k;
+        const bool pred = (t + 1 < F) && (frac < incl[k] / total);
+        const uint64_t bal = __ballot(pred);
+        const uint64_t bits = (bal >> gbase) & (SG == 64 ? ~0ull : ((1ull << SG) - 1));
+        if (!found && bits) {
+            res = SG * k + __builtin_ctzll(bits);
+            found = true;
+        }
+    }
+    // clamp for a partial last leaf (partition_tree.hpp:353, 373-391)
+    if (res >= rows) res = rows - 1;
+    return res;
+}
+
+template <int RMAX, int SG, int BS, bool USE32>
+__global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
+    constexpr int GPW = 64 / SG;  // groups per wave
+    __shared__ double lds_x[BS / SG][RMAX];
+    const int lane = threadIdx.x & 63;
+    const int g = lane % SG;
+    const int wave = threadIdx.x / 64;
+    const int grp_in_block = threadIdx.x / SG;
+    double* xs = lds_x[grp_in_block];
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    const int epl = (T + SG - 1) / SG;
+    constexpr bool fp32 = USE32;
+    // entries R..RMAX-1 of x stay zero (row_dot's generic path reads them)
+    for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
+    wave_sync_lds();
+
+    const uint64_t waves_total = (uint64_t)gridDim.x * (BS / 64);
+    const uint64_t wave_id = (uint64_t)blockIdx.x * (BS / 64) + wave;
+    const uint64_t per_wave = GPW;
+    const uint64_t n_items = (a.J + per_wave - 1) / per_wave;
+
+    for (uint64_t item = wave_id; item < n_items; item += waves_total) {
+        const uint64_t i = item * per_wave + lane / SG;
+        const bool active = i < a.J;
+        double h[HPL];
+#pragma unroll
+        for (int k = 0; k < HPL; k++) h[k] = 1.0;
+
+        for (int mi = 0; mi < a.m; mi++) {
+            const int k = a.order[mi];
+            const ModeDev& md = a.mode[k];
+            // ---- eigen stage: x = h -----------------------------------------
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) xs[g + SG * q] = h[q];
+            wave_sync_lds();
+            double c64[EPL];
+            float c32[EPL];
+            make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+            double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i]
+                                          : philox_uniform(a.seed, a.call, i, 2 * k))
+                               : 0.5;
+            double low, high;
+            const uint64_t ce = descend<RMAX, SG>(md, a.esh, a.ts64, a.ts32, c64, c32, false, md.etree,
+                                                  ue, g, epl, T, low, high);
+            const uint64_t r = leaf_index(a.esh, ce);
+            wave_sync_lds();
+            // scaled_h = h (.) E_k[r]
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) xs[g + SG * q] = h[q] * md.E[r * R + g + SG * q];
+            wave_sync_lds();
+            // ---- Gram stage: x = scaled_h -------------------------------------
+            make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+            if constexpr (fp32) {
+#pragma unroll
+                for (int q = 0; q < EPL; q++) c32[q] = (float)c64[q];
+            }
+            const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                                : philox_uniform(a.seed, a.call, i, 2 * k + 1))
+                                     : 0.5;
+            const uint64_t c = descend<RMAX, SG>(md, md.sh, a.ts64, a.ts32, c64, c32, fp32, md.tree64, ug,
+                                                 g, epl, T, low, high);
+            const uint64_t idx = leaf_index(md.sh, c);
+            uint32_t res = 0;
+            if (md.sh.F > 1) {
+                const double frac = (ug - low) / (high - low);
+                const uint64_t r0 = idx * md.sh.F;
+                const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
+                if constexpr (fp32)
+                    res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, frac, g, lane);
+                else
+                    res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, frac, g,
+                                                      lane);
+            }
+            const uint64_t srow = idx * md.sh.F + res;
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) h[q] *= md.U64[srow * R + g + SG * q];
+            if (active && g == 0) a.samples[(uint64_t)k * a.J + i] = srow;
+            wave_sync_lds();
+        }
+        // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = h[q];
+        wave_sync_lds();
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < HPL; q++) {
+            const int ra = g + SG * q;
+            if (ra < R) {
+                double y = 0.0;
+                for (int b = 0; b < R; b++) y += a.pinv[ra * R + b] * xs[b];
+                part += h[q] * y;
+            }
+        }
+        const double lev = group_sum<SG>(part);
+        if (active) {
+            if (a.w_out && g == 0) a.w_out[i] = (double)R / (lev * (double)a.J);
+            if (a.h_out) {
+#pragma unroll
+                for (int q = 0; q < HPL; q++)
+                    if (g + SG * q < R) a.h_out[i * R + g + SG * q] = h[q];
+            }
+        }
+        wave_sync_lds();
+    }
+}
+
+}  // namespace ftl

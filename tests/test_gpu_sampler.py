@@ -1,0 +1,168 @@
+"""Parity of the HIP sampler (libftl_hip.so) with the CPU FP64 oracle on
+identical inputs and identical uniforms.
+
+Bar: FP64 trees and computeM outputs bit-exact; sampled indices identical
+wherever the oracle's margin (distance of a uniform to the CDF boundary it was
+compared with) exceeds 1e-10; h identical and weights within 1e-10 relative.
+FP32 mode: index mismatch rate vs the FP64 oracle reported and bounded.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from tests.helpers import exact_leverage, flat_index, multinomial_tv_band, philox_draws, tv_distance
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 1e-10
+
+CASES = [
+    # id, dims, R, J, j
+    ("c1_readme", [1000] * 4, 8, 10_000, 5),
+    ("dist_cmp", [8, 8, 8], 8, 50_000, 4),
+    ("r32_i4096", [4096] * 3, 32, 20_000, 4),
+    ("ragged_r25_excl1", [3000, 2500, 1777], 25, 10_000, 1),
+    ("r16_excl_last", [5000, 4000], 16, 8_000, 1),
+    ("r64_excl0", [2048, 1000, 3000], 64, 4_000, 0),
+    ("r128", [700, 300], 128, 1_000, 2),
+    ("n5_r8", [50, 60, 70, 80, 90], 8, 5_000, 2),
+    ("n1", [1000], 8, 1_000, 1),
+    ("tiny_rows", [1, 5, 7], 4, 1_000, 3),
+]
+
+
+def _factors(dims, R, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal((I, R)) for I in dims]
+
+
+@pytest.fixture(scope="module")
+def ftl():
+    import fast_tensor_leverage_amd as ftl
+    return ftl
+
+
+@pytest.mark.parametrize("cid,dims,R,J,j", CASES, ids=[c[0] for c in CASES])
+def test_fp64_parity(ftl, oracle, cid, dims, R, J, j):
+    print(f"[gpu] fp64 parity case={cid} dims={dims} R={R} J={J} j={j}", flush=True)
+    U = _factors(dims, R, seed=zlib.crc32(cid.encode()))
+    N = len(dims)
+    ref = oracle.OracleSampler(U, J, R)
+    gpu = ftl.Sampler(U, J, R, "efficient", precision="fp64")
+    # trees: bit-exact
+    for k in range(N):
+        np.testing.assert_array_equal(gpu.get_tree(k), ref.tree(k))
+    # computeM: pinv and eigen matrices
+    p_ref, E_ref = ref.compute_m(j)
+    p_gpu, E_gpu = gpu.compute_m(j)
+    iu = np.triu_indices(R)
+    np.testing.assert_allclose(p_gpu[iu], p_ref[iu], rtol=1e-12, atol=1e-14 * np.abs(p_ref).max())
+    for k in range(N):
+        if j < N and k == j:
+            continue
+        np.testing.assert_allclose(np.abs(E_gpu[k]), np.abs(E_ref[k]), rtol=1e-9,
+                                   atol=1e-12 * np.abs(E_ref[k]).max())
+    draws = np.random.default_rng(7).random(2 * N * J)
+    out = ref.draw(j, draws)
+    samples = np.zeros((N, J), dtype=np.uint64)
+    h = np.zeros((J, R))
+    w = np.zeros(J)
+    gpu.KRPDrawSamples_materialize(j, samples, h, w, draws)
+    modes = [k for k in range(N) if not (j < N and k == j)]
+    sure = out["margins"] > MARGIN
+    diff = np.zeros(J, dtype=bool)
+    for k in modes:
+        diff |= samples[k] != out["samples"][k]
+    print(f"[gpu] {cid}: mismatches {int(diff.sum())} / {J}, near-boundary {int((~sure).sum())}", flush=True)
+    assert not np.any(diff & sure), np.flatnonzero(diff & sure)[:10]
+    same = ~diff
+    np.testing.assert_array_equal(h[same], out["h"][same])
+    np.testing.assert_allclose(w[same], out["weights"][same], rtol=1e-10)
+    if j < N:
+        assert np.all(samples[j] == 0)
+
+
+@pytest.mark.parametrize("cid,dims,R,J,j", CASES[:7], ids=[c[0] for c in CASES[:7]])
+def test_fp32_mismatch_rate(ftl, oracle, cid, dims, R, J, j):
+    print(f"[gpu] fp32 case={cid}", flush=True)
+    U = _factors(dims, R, seed=zlib.crc32(cid.encode()))
+    N = len(dims)
+    ref = oracle.OracleSampler(U, J, R)
+    gpu = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    draws = np.random.default_rng(11).random(2 * N * J)
+    out = ref.draw(j, draws)
+    samples = np.zeros((N, J), dtype=np.uint64)
+    gpu.KRPDrawSamples(j, samples, draws)
+    modes = [k for k in range(N) if not (j < N and k == j)]
+    diff = np.zeros(J, dtype=bool)
+    for k in modes:
+        diff |= samples[k] != out["samples"][k]
+    rate = diff.mean()
+    print(f"[gpu] {cid}: fp32 mismatch rate {rate:.2e}", flush=True)
+    assert rate <= 2e-3
+
+
+def test_builtin_generator_matches_philox(ftl, oracle):
+    """draws=None uses Philox4x32-10; the NumPy restatement reproduces it."""
+    dims, R, J = [300, 200, 100], 8, 4000
+    U = _factors(dims, R, 5)
+    gpu = ftl.Sampler(U, J, R)
+    gpu.set_seed(1234, 5)
+    samples = np.zeros((3, J), dtype=np.uint64)
+    gpu.KRPDrawSamples(4, samples)
+    draws = philox_draws(1234, 5, 3, J)
+    out = oracle.OracleSampler(U, J, R).draw(4, draws)
+    sure = out["margins"] > MARGIN
+    assert np.array_equal(samples[:, sure], out["samples"][:, sure])
+    # the counter advances: a second call draws different uniforms
+    s2 = np.zeros_like(samples)
+    gpu.KRPDrawSamples(4, s2)
+    assert not np.array_equal(s2, samples)
+
+
+def test_distribution_tv(ftl):
+    """Empirical TV to exact leverage within the multinomial noise band
+    (the check of compare_distributions.py, made quantitative)."""
+    N, I, R, J = 3, 8, 8, 200_000
+    U = _factors([I] * N, R, 2024)
+    p = exact_leverage(U)
+    mean, std = multinomial_tv_band(p, J)
+    for prec in ("fp64", "fp32"):
+        gpu = ftl.Sampler(U, J, R, precision=prec)
+        samples = np.zeros((N, J), dtype=np.uint64)
+        gpu.KRPDrawSamples(N + 1, samples)
+        emp = np.bincount(flat_index(samples, [I] * N), minlength=p.size) / J
+        tv = tv_distance(emp, p)
+        print(f"[gpu] TV {prec}: {tv:.4f} (band {mean:.4f} +- {std:.4f})", flush=True)
+        assert abs(tv - mean) < 5 * std + 1e-3
+
+
+def test_update_sampler(ftl, oracle):
+    dims, R, J = [500, 400, 300], 16, 2000
+    U = _factors(dims, R, 9)
+    gpu = ftl.Sampler(U, J, R)
+    new = np.random.default_rng(10).standard_normal((400, R))
+    gpu.update_sampler(1, new)
+    ref = oracle.OracleSampler([U[0], new, U[2]], J, R)
+    for k in range(3):
+        np.testing.assert_array_equal(gpu.get_tree(k), ref.tree(k))
+    draws = np.random.default_rng(1).random(6 * J)
+    out = ref.draw(4, draws)
+    samples = np.zeros((3, J), dtype=np.uint64)
+    gpu.KRPDrawSamples(4, samples, draws)
+    sure = out["margins"] > MARGIN
+    assert np.array_equal(samples[:, sure], out["samples"][:, sure])
+
+
+def test_argument_errors(ftl):
+    U = _factors([100, 100], 4, 1)
+    gpu = ftl.Sampler(U, 10, 4)
+    with pytest.raises(TypeError):
+        gpu.KRPDrawSamples(2, np.zeros((2, 10), dtype=np.int64))
+    with pytest.raises(ValueError):
+        gpu.KRPDrawSamples(2, np.zeros((3, 10), dtype=np.uint64))
+    with pytest.raises(ValueError):
+        ftl.Sampler([np.zeros((10, 4))], 10, 4).KRPDrawSamples(0, np.zeros((1, 10), dtype=np.uint64))
+    with pytest.raises(ValueError):
+        gpu.update_sampler(5)
