@@ -35,7 +35,8 @@ SIGNATURES = {
     "ftl_sampler_compute_m": ([_vp, _u32, _vp, _vp], _i32),
     "ftl_sampler_info": ([_vp, _vp], _i32),
     "ftl_sampler_stream": ([_vp], _vp),
-    "ftl_sampler_set_tuning": ([_vp, _u32, _u32], _i32),
+    "ftl_sampler_set_option": ([_vp, ctypes.c_char_p, ctypes.c_int64], _i32),
+    "ftl_sampler_get_option": ([_vp, ctypes.c_char_p, _vp], _i32),
     "ftl_sampler_enable_timing": ([_vp, _i32], _i32),
     "ftl_sampler_last_timing": ([_vp, _vp], _i32),
 }

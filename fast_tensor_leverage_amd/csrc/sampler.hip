@@ -32,6 +32,7 @@ struct ModeHost {
     float* U32 = nullptr;
     double* tree64 = nullptr;
     float* tree32 = nullptr;
+    uint64_t tree32_elems = 0;
 };
 
 __global__ void k_convert_f32(const double* in, float* out, uint64_t n) {
@@ -57,6 +58,7 @@ struct Sampler {
     int precision = FTL_FP64;
     uint32_t ts64 = 0, ts32 = 0;
     uint32_t fp32_levels = kDefaultFp32Levels;
+    uint64_t tree_pad = 0;  // zeroed entries after every tree (draw kernel over-read)
     std::vector<ModeHost> modes;
     double *pinv = nullptr, *W = nullptr, *E = nullptr, *etree = nullptr, *work = nullptr, *lam = nullptr;
     uint32_t etree_nodes = 0;
@@ -173,13 +175,58 @@ struct Sampler {
             const uint64_t nel = m.I * R;
             hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, stream, m.U64, m.U32, nel);
             FTL_HIP_CHECK(hipGetLastError());
-            if (m.sh.node_count > m.n64) {
+            if (m.tree32) {
                 const uint64_t work = (m.sh.node_count - m.n64) * ts32;
                 hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(work, 256)), dim3(256), 0, stream, m.tree64,
                                    m.tree32, m.n64, m.sh.node_count, ts64, ts32);
                 FTL_HIP_CHECK(hipGetLastError());
             }
         }
+    }
+
+    // (Re)allocate and fill the FP32 copy of the levels below d64 of mode k.
+    void make_fp32_copy(uint32_t k) {
+        ModeHost& m = modes[k];
+        if (m.tree32) {
+            dfree(m.tree32, m.tree32_elems * sizeof(float));
+            m.tree32 = nullptr;
+            m.tree32_elems = 0;
+        }
+        set_mode_precision(m);
+        if (precision != FTL_FP32 || m.sh.node_count <= m.n64) return;
+        const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
+        m.tree32_elems = n32 + tree_pad;
+        m.tree32 = dalloc<float>(m.tree32_elems);
+        FTL_HIP_CHECK(hipMemsetAsync(m.tree32 + n32, 0, tree_pad * sizeof(float), stream));
+        hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(n32, 256)), dim3(256), 0, stream, m.tree64, m.tree32, m.n64,
+                           m.sh.node_count, ts64, ts32);
+        FTL_HIP_CHECK(hipGetLastError());
+    }
+
+    void set_option(const std::string& key, int64_t value) {
+        if (key == "group_lanes") {
+            FTL_REQUIRE(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
+                        "group_lanes must be 0, 8, 16, 32 or 64");
+            sg_override = (uint32_t)value;
+        } else if (key == "grid_blocks") {
+            FTL_REQUIRE(value >= 0, "grid_blocks must be >= 0");
+            grid_override = (uint32_t)value;
+        } else if (key == "fp32_levels") {
+            FTL_REQUIRE(value >= 0 && value <= 64, "fp32_levels must be in [0, 64]");
+            fp32_levels = value == 0 ? kDefaultFp32Levels : (uint32_t)value;
+            for (uint32_t k = 0; k < N; k++) make_fp32_copy(k);
+            FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        } else {
+            throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
+        }
+    }
+
+    int64_t get_option(const std::string& key) const {
+        if (key == "group_lanes") return sg_override;
+        if (key == "grid_blocks") return grid_override;
+        if (key == "fp32_levels") return fp32_levels;
+        if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
+        throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
     }
 
     void init(uint32_t N_, const uint64_t* I, uint64_t J_, uint64_t R_, const double* const* U, bool on_device,
@@ -201,12 +248,20 @@ struct Sampler {
         if (!attrs_set) {
             FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_leaf_gram<256>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_gram_pinv<256>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
-            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_eigen_modes<256>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+            // computeM blocks hold A ((R+1) R doubles) in LDS, plus V and the
+            // eigenvectors when R <= 64: at most 132 KB of dynamic LDS
+            const void* cm[] = {(const void*)k_gram_pinv<kJacobiThreads>, (const void*)k_eigen_modes<kJacobiThreads>};
+            for (const void* f : cm)
+                FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
             attrs_set = true;
         }
+        // Tail pad of every tree: the draw kernel reads SG * ceil(Tmax/SG)
+        // entries from a node start, Tmax being the packed size for the
+        // kernel's compile-time rank bound (R rounded up to a power of two).
+        uint64_t rmax = 8;
+        while (rmax < R) rmax *= 2;
+        const uint64_t pad = tri_size(rmax) + 64;
+        tree_pad = pad;
         modes.resize(N);
         for (uint32_t k = 0; k < N; k++) {
             FTL_REQUIRE(I[k] >= 1 && I[k] < (1ull << 40), "factor row counts must be in [1, 2^40)");
@@ -218,10 +273,16 @@ struct Sampler {
             m.U64 = dalloc<double>(m.I * R);
             FTL_HIP_CHECK(hipMemcpyAsync(m.U64, U[k], m.I * R * sizeof(double),
                                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
-            m.tree64 = dalloc<double>(m.sh.node_count * ts64);
+            m.tree64 = dalloc<double>(m.sh.node_count * ts64 + pad);
+            FTL_HIP_CHECK(hipMemsetAsync(m.tree64 + m.sh.node_count * ts64, 0, pad * sizeof(double), stream));
             if (precision == FTL_FP32) {
                 m.U32 = dalloc<float>(m.I * R);
-                if (m.sh.node_count > m.n64) m.tree32 = dalloc<float>((m.sh.node_count - m.n64) * ts32);
+                if (m.sh.node_count > m.n64) {
+                    const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
+                    m.tree32_elems = n32 + pad;
+                    m.tree32 = dalloc<float>(m.tree32_elems);
+                    FTL_HIP_CHECK(hipMemsetAsync(m.tree32 + n32, 0, pad * sizeof(float), stream));
+                }
             }
             build_mode(k);
         }
@@ -230,11 +291,11 @@ struct Sampler {
         pinv = dalloc<double>(R2);
         W = dalloc<double>(R2);
         E = dalloc<double>(N * R2);
-        etree = dalloc<double>((uint64_t)N * etree_nodes * ts64);
-        work = dalloc<double>((uint64_t)(N + 1) * 2 * R2);
+        etree = dalloc<double>((uint64_t)N * etree_nodes * ts64 + pad);
+        work = dalloc<double>((uint64_t)(N + 1) * 3 * (R + 1) * R);
         lam = dalloc<double>((uint64_t)(N + 1) * R);
         FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R2 * sizeof(double), stream));
-        FTL_HIP_CHECK(hipMemsetAsync(etree, 0, (uint64_t)N * etree_nodes * ts64 * sizeof(double), stream));
+        FTL_HIP_CHECK(hipMemsetAsync(etree, 0, ((uint64_t)N * etree_nodes * ts64 + pad) * sizeof(double), stream));
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -275,12 +336,14 @@ struct Sampler {
         std::vector<int> order = sampled_modes(j);
         FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
         ComputeMArgs a = computeM_args(order);
-        const size_t lds = 2 * R * R <= 8192 ? 2 * R * R * sizeof(double) : 0;
+        const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
+        const size_t lds = jac * sizeof(double);
+        constexpr int BS = kJacobiThreads;
         if (evs) FTL_HIP_CHECK(hipEventRecord(evs[0], st));
-        hipLaunchKernelGGL(k_gram_pinv<256>, dim3(1), dim3(256), lds, st, a);
+        hipLaunchKernelGGL(k_gram_pinv<BS>, dim3(1), dim3(BS), lds, st, a);
         FTL_HIP_CHECK(hipGetLastError());
         if (evs) FTL_HIP_CHECK(hipEventRecord(evs[1], st));
-        hipLaunchKernelGGL(k_eigen_modes<256>, dim3(a.m), dim3(256), lds, st, a);
+        hipLaunchKernelGGL(k_eigen_modes<BS>, dim3(a.m), dim3(BS), lds, st, a);
         FTL_HIP_CHECK(hipGetLastError());
         if (evs) FTL_HIP_CHECK(hipEventRecord(evs[2], st));
     }
@@ -311,9 +374,9 @@ struct Sampler {
         } else if (R <= 16) {
             launch_draw_p<16, 16>(a, st);
         } else if (R <= 32) {
-            // measured at N=3, I=2^23, R=32, J=5e4: SG=32 fastest for FP64,
-            // SG=64 for FP32
-            const uint32_t s = sg ? sg : (precision == FTL_FP32 ? 64u : 32u);
+            // measured at N=3, I=2^23, R=32, J=5e4: SG=32 fastest for both
+            // precisions (FP64 1.56 ms vs 1.64 at SG=16; FP32 1.15 vs 1.33)
+            const uint32_t s = sg ? sg : 32u;
             if (s == 8) launch_draw_p<32, 8>(a, st);
             else if (s == 16) launch_draw_p<32, 16>(a, st);
             else if (s == 64) launch_draw_p<32, 64>(a, st);
@@ -550,14 +613,17 @@ int ftl_sampler_info(ftl_sampler* s, uint64_t* out) {
 
 void* ftl_sampler_stream(ftl_sampler* s) { return s ? (void*)s->impl.stream : nullptr; }
 
-int ftl_sampler_set_tuning(ftl_sampler* s, uint32_t group_lanes, uint32_t grid_blocks) {
+int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value) {
     return guarded([&] {
-        FTL_REQUIRE(s, "null sampler");
-        FTL_REQUIRE(group_lanes == 0 || group_lanes == 8 || group_lanes == 16 || group_lanes == 32 ||
-                        group_lanes == 64,
-                    "group_lanes must be 0, 8, 16, 32 or 64");
-        s->impl.sg_override = group_lanes;
-        s->impl.grid_override = grid_blocks;
+        FTL_REQUIRE(s && key, "null argument");
+        s->impl.set_option(key, value);
+    });
+}
+
+int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value) {
+    return guarded([&] {
+        FTL_REQUIRE(s && key && value, "null argument");
+        *value = s->impl.get_option(key);
     });
 }
 

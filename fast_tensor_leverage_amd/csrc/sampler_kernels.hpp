@@ -137,106 +137,181 @@ __global__ void k_tree_to_f32(const double* tree64, float* tree32, uint64_t n64,
 
 // ---------------------------------------------------------------------------
 // Cyclic Jacobi eigensolver for one symmetric R x R matrix held by a block
-// (A and V in LDS or global scratch).  Round-robin pair order; within a round
-// all rotations are computed from the same A, then all row updates, then all
-// column updates (A and V).  Rotation annihilating A[p][q] (Golub & Van Loan
-// Alg. 8.5.2), skipped when |A[p][q]| <= eps*sqrt(|A[p][p] A[q][q]|).
-// Products and sums are rounded individually (no contraction).  On return lam
-// is ascending and Vout column v holds the eigenvector of lam[v].
+// (A in LDS, V in LDS or global scratch).  Round-robin pair order; a round
+// first computes its R/2 rotations from the current A (one thread each),
+// then applies J^T A J in place: each thread owns the 2x2 blocks where a row
+// pair meets a column pair and applies the row rotation, then the column
+// rotation (the same operations, in the same order, as applying all row
+// rotations and then all column rotations); V is rotated in place.  Rotation
+// annihilating A[p][q] (Golub & Van Loan Alg. 8.5.2), skipped when
+// |A[p][q]| <= eps*sqrt(|A[p][p] A[q][q]|).  Products and sums are rounded
+// individually (no contraction).  On return lam is ascending and Vout column
+// v holds the eigenvector of lam[v].
 // ---------------------------------------------------------------------------
 struct JacobiShared {
     int p[64], q[64], act[64];
     double c[64], s[64];
 };
 
+// 1/x and 1/sqrt(x): v_rcp_f64 / v_rsq_f64 estimates plus one Newton step.
+__device__ inline double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ inline double fast_rsq(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * r;
+    const double e = fma(-h, r, 0.5);
+    return fma(r, e, r);
+}
+
+// Pair k of round `round` in the circle-method ordering of n2 (even) indices:
+// position 0 fixed, positions 1..n2-1 rotating; returned with p < q.
+__device__ inline void jacobi_pair(int k, int round, int n2, int& p, int& q) {
+    p = k == 0 ? 0 : 1 + ((k - 1 + round) % (n2 - 1));
+    const int kk = n2 - 1 - k;
+    q = kk == 0 ? 0 : 1 + ((kk - 1 + round) % (n2 - 1));
+    if (p > q) {
+        const int t = p;
+        p = q;
+        q = t;
+    }
+}
+
+// Rotation (c, s) annihilating A[p][q]; false (c = 1, s = 0) when the entry
+// is negligible or q is the padding index of an odd size.
+__device__ inline bool jacobi_rotation(const double* A, int LD, int R, int p, int q, double& c, double& s) {
+#pragma clang fp contract(off)
+    c = 1.0;
+    s = 0.0;
+    if (q >= R) return false;
+    const double eps = 2.220446049250313e-16;
+    const double apq = A[p * LD + q], app = A[p * LD + p], aqq = A[q * LD + q];
+    // |apq| <= eps sqrt(|app aqq|), compared in squares (no square root on
+    // the round's critical path)
+    if (apq * apq <= (eps * eps) * fabs(app * aqq) || apq == 0.0) return false;
+    // hardware reciprocal / reciprocal square root, each refined by one
+    // Newton step (error ~1 ulp; the rotation stays orthogonal to working
+    // precision)
+    const double tau = (aqq - app) * fast_rcp(2.0 * apq);
+    double t;
+    if (fabs(tau) > 1e150) {
+        t = 0.5 * fast_rcp(tau);
+    } else {
+        const double x = 1.0 + tau * tau;
+        const double sq = x * fast_rsq(x);
+        t = (tau >= 0 ? 1.0 : -1.0) * fast_rcp(fabs(tau) + sq);
+    }
+    c = fast_rsq(1.0 + t * t);
+    s = t * c;
+    return true;
+}
+
 template <int BS>
-__device__ void jacobi_block(double* A, double* V, int R, double* lam, double* Vout,
-                             JacobiShared& js, double* diag_tmp) {
+__device__ void jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
+                             double* diag_tmp) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
+    const int LD = R + 1;  // odd leading dimension: rows start in different LDS banks
     const int n2 = R + (R & 1);
     const int npairs = n2 / 2;
-    const double eps = 2.220446049250313e-16;
-    for (int e = tid; e < R * R; e += BS) V[e] = (e / R == e % R) ? 1.0 : 0.0;
+    for (int e = tid; e < R * R; e += BS) V[(e / R) * LD + e % R] = (e / R == e % R) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 40; sweep++) {
         int any = 0;
         for (int round = 0; round + 1 < n2; round++) {
+            // rotations of this round, each computed once from the current A
             if (tid < npairs) {
-                const int k = tid;
-                int p = k == 0 ? 0 : 1 + ((k - 1 + round) % (n2 - 1));
-                const int kk = n2 - 1 - k;
-                int q = kk == 0 ? 0 : 1 + ((kk - 1 + round) % (n2 - 1));
-                if (p > q) {
-                    int t = p;
-                    p = q;
-                    q = t;
-                }
-                int act = 0;
-                double c = 1.0, s = 0.0;
-                if (q < R) {
-                    const double apq = A[p * R + q], app = A[p * R + p], aqq = A[q * R + q];
-                    const double thr = eps * sqrt(fabs(app * aqq));
-                    if (!(fabs(apq) <= thr || apq == 0.0)) {
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        double t;
-                        if (fabs(tau) > 1e150) {
-                            t = 0.5 / tau;
-                        } else {
-                            const double sq = sqrt(1.0 + tau * tau);
-                            t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sq);
-                        }
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        s = t * c;
-                        act = 1;
-                    }
-                }
-                js.p[k] = p;
-                js.q[k] = q;
-                js.c[k] = c;
-                js.s[k] = s;
-                js.act[k] = act;
+                int p, q;
+                double c, s;
+                jacobi_pair(tid, round, n2, p, q);
+                const bool act = jacobi_rotation(A, LD, R, p, q, c, s);
+                js.p[tid] = p;
+                js.q[tid] = q;
+                js.c[tid] = c;
+                js.s[tid] = s;
+                js.act[tid] = act;
                 any |= act;
             }
             __syncthreads();
-            for (int it = tid; it < npairs * R; it += BS) {
-                const int k = it / R, j = it % R;
-                if (!js.act[k]) continue;
-                const int p = js.p[k], q = js.q[k];
-                const double c = js.c[k], s = js.s[k];
-                const double ap = A[p * R + j], aq = A[q * R + j];
-                A[p * R + j] = c * ap - s * aq;
-                A[q * R + j] = s * ap + c * aq;
+            // J^T A J in place: the 2x2 block where row pair k meets column
+            // pair k2 belongs to one thread, which applies the row rotation
+            // and then the column rotation (the operation order of applying
+            // all row rotations, then all column rotations)
+            for (int it = tid; it < npairs * npairs; it += BS) {
+                const int k = it / npairs, k2 = it - (it / npairs) * npairs;
+                const bool act1 = js.act[k], act2 = js.act[k2];
+                if (!act1 && !act2) continue;
+                const int p = js.p[k], q = js.q[k], p2 = js.p[k2], q2 = js.q[k2];
+                const double c = js.c[k], s = js.s[k], c2 = js.c[k2], s2 = js.s[k2];
+                // an index of value R (padding of an odd size) only occurs in
+                // inactive pairs; its entries are treated as absent
+                const bool qv = q < R, q2v = q2 < R;
+                const double app2 = A[p * LD + p2];
+                const double apq2 = q2v ? A[p * LD + q2] : 0.0;
+                const double aqp2 = qv ? A[q * LD + p2] : 0.0;
+                const double aqq2 = qv && q2v ? A[q * LD + q2] : 0.0;
+                double rpp2 = app2, rpq2 = apq2, rqp2 = aqp2, rqq2 = aqq2;
+                if (act1) {
+                    rpp2 = c * app2 - s * aqp2;
+                    rqp2 = s * app2 + c * aqp2;
+                    rpq2 = c * apq2 - s * aqq2;
+                    rqq2 = s * apq2 + c * aqq2;
+                }
+                double npp2 = rpp2, npq2 = rpq2, nqp2 = rqp2, nqq2 = rqq2;
+                if (act2) {
+                    npp2 = c2 * rpp2 - s2 * rpq2;
+                    npq2 = s2 * rpp2 + c2 * rpq2;
+                    nqp2 = c2 * rqp2 - s2 * rqq2;
+                    nqq2 = s2 * rqp2 + c2 * rqq2;
+                }
+                A[p * LD + p2] = npp2;
+                if (q2v) A[p * LD + q2] = npq2;
+                if (qv) A[q * LD + p2] = nqp2;
+                if (qv && q2v) A[q * LD + q2] = nqq2;
             }
-            __syncthreads();
-            for (int it = tid; it < npairs * R; it += BS) {
-                const int k = it / R, i = it % R;
-                if (!js.act[k]) continue;
-                const int p = js.p[k], q = js.q[k];
-                const double c = js.c[k], s = js.s[k];
-                const double ap = A[i * R + p], aq = A[i * R + q];
-                A[i * R + p] = c * ap - s * aq;
-                A[i * R + q] = s * ap + c * aq;
-                const double vp = V[i * R + p], vq = V[i * R + q];
-                V[i * R + p] = c * vp - s * vq;
-                V[i * R + q] = s * vp + c * vq;
+            // V J
+            for (int it = tid; it < R * npairs; it += BS) {
+                const int i = it / npairs, k2 = it - (it / npairs) * npairs;
+                if (!js.act[k2]) continue;
+                const int p = js.p[k2], q = js.q[k2];
+                const double c = js.c[k2], s = js.s[k2];
+                const double vp = V[i * LD + p], vq = V[i * LD + q];
+                V[i * LD + p] = c * vp - s * vq;
+                V[i * LD + q] = s * vp + c * vq;
             }
             __syncthreads();
         }
         if (!__syncthreads_or(any)) break;
+        // converged when every pair meets the skip test: the next sweep would
+        // rotate nothing, so stopping here gives the same result
+        int open = 0;
+        for (int w = tid; w < R * R; w += BS) {
+            const int p = w / R, q = w - (w / R) * R;
+            if (p < q) {
+                const double apq = A[p * LD + q];
+                const double thr = (2.220446049250313e-16 * 2.220446049250313e-16) * fabs(A[p * LD + p] * A[q * LD + q]);
+                open |= !(apq * apq <= thr || apq == 0.0);
+            }
+        }
+        if (!__syncthreads_or(open)) break;
     }
-    // stable ascending sort of the diagonal
-    for (int v = tid; v < R; v += BS) diag_tmp[v] = A[v * R + v];
+    // stable ascending sort of the diagonal; Vout (R x R, dense) column v is
+    // the eigenvector of lam[v]
+    for (int v = tid; v < R; v += BS) diag_tmp[v] = A[v * LD + v];
     __syncthreads();
-    for (int v = tid; v < R; v += BS) {
+    for (int w = tid; w < R * R; w += BS) {
+        const int v = w / R, i = w - v * R;
         const double d = diag_tmp[v];
         int rank = 0;
-        for (int w = 0; w < R; w++) {
-            const double dw = diag_tmp[w];
-            rank += (dw < d) || (dw == d && w < v);
+        for (int u = 0; u < R; u++) {
+            const double du = diag_tmp[u];
+            rank += (du < d) || (du == d && u < v);
         }
-        lam[rank] = d;
-        for (int i = 0; i < R; i++) Vout[i * R + rank] = V[i * R + v];
+        if (i == 0) lam[rank] = d;
+        Vout[i * R + rank] = V[i * LD + v];
     }
     __syncthreads();
 }
@@ -253,7 +328,7 @@ struct ComputeMArgs {
     double* W;       // R x R, columns = eigvecs / sqrt(lambda), descending lambda
     double* E;       // N x R x R
     double* etree;   // N x etree_nodes x ts64
-    double* work;    // per block: 2 * R * R doubles (A, V) when they do not fit LDS
+    double* work;    // per block: 3 (R+1) R doubles (V and eigenvectors when they do not fit LDS)
     double* lam_out; // (m+1) x R eigenvalues (diagnostics)
     uint32_t ts64;
     uint32_t etree_nodes;
@@ -263,7 +338,42 @@ __device__ inline double packed_get(const double* P, int R, int a, int b) {
     return a <= b ? P[tri_offset(R, a, b)] : P[tri_offset(R, b, a)];
 }
 
-// Block-level A/V storage: LDS when 2 R^2 doubles fit, global scratch otherwise.
+// Jacobi storage of a block: A ((R+1) R doubles, rows padded to an odd
+// length) always in LDS; V ((R+1) R) and the sorted eigenvectors (R^2) in LDS
+// as well when all three fit in kJacobiLdsDoubles (R <= 64), otherwise in the
+// block's global scratch (R = 128).
+constexpr int kJacobiLdsDoubles = 3 * 65 * 64;
+constexpr int kJacobiThreads = 1024;
+
+struct JacobiStore {
+    double *A, *V, *vecs;
+};
+
+__device__ inline JacobiStore jacobi_store(double* smem, double* work, int R) {
+    const int LD = R + 1;
+    JacobiStore st;
+    st.A = smem;
+    if (3 * LD * R <= kJacobiLdsDoubles) {
+        st.V = smem + LD * R;
+        st.vecs = smem + 2 * LD * R;
+    } else {
+        st.V = work;
+        st.vecs = work + LD * R;
+    }
+    return st;
+}
+
+// Packed-entry -> (row, col) table of the upper triangle (entry e < T).
+__device__ inline void fill_pair_table(uint8_t* pa, uint8_t* pb, int R) {
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        const int base = r * R - r * (r - 1) / 2;
+        for (int c = r; c < R; c++) {
+            pa[base + c - r] = (uint8_t)r;
+            pb[base + c - r] = (uint8_t)c;
+        }
+    }
+}
+
 template <int BS>
 __global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
 #pragma clang fp contract(off)
@@ -271,26 +381,33 @@ __global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
     __shared__ JacobiShared js;
     __shared__ double diag_tmp[128];
     const int R = a.R, R2 = R * R;
-    const bool in_lds = 2 * R2 <= 8192;
-    double* A = in_lds ? smem : a.work;
-    double* V = in_lds ? smem + R2 : a.work + R2;
+    const int LD = R + 1;
+    const JacobiStore st = jacobi_store(smem, a.work, R);
+    double* A = st.A;
     double* lam = a.lam_out;
-    double* Vs = a.pinv;  // sorted eigenvectors parked in the pinv buffer first
-    for (int e = threadIdx.x; e < R2; e += BS) {
-        const int r = e / R, c = e % R;
-        double g = 1.0;
-        // suffix order of the reference: G_{m-1} (.) ... (.) G_0 applied
-        // from the last sampled mode down (efficient_krp_sampler.hpp:95-105)
-        for (int t = a.m - 1; t >= 0; t--) g = packed_get(a.roots[a.modes[t]], R, r, c) * g;
-        A[e] = g;
+    // G = Hadamard product of the sampled root Grams, multiplied in the
+    // reference's suffix order G_{m-1} (.) ... (.) G_0 (efficient_krp_sampler.hpp:95-105);
+    // each pass issues its loads for all entries of the thread at once.
+    for (int e = threadIdx.x; e < R2; e += BS) A[(e / R) * LD + e % R] = 1.0;
+    for (int t = a.m - 1; t >= 0; t--) {
+        const double* G = a.roots[a.modes[t]];
+        for (int e = threadIdx.x; e < R2; e += BS) {
+            const int r = e / R, c = e % R;
+            A[r * LD + c] = packed_get(G, R, r, c) * A[r * LD + c];
+        }
     }
     __syncthreads();
-    jacobi_block<BS>(A, V, R, lam, Vs, js, diag_tmp);
-    // W[u][R-1-v] = V[u][v] / sqrt(lam[v]) when lam[v] > 1e-8 (126-138)
+    jacobi_block<BS>(A, st.V, R, lam, st.vecs, js, diag_tmp);
+    const double* vecs = st.vecs;
+    // W[u][R-1-v] = V[u][v] / sqrt(lam[v]) when lam[v] > 1e-8 (126-138); W is
+    // staged in the (now free) V buffer and copied out for the last mode's E.
+    double* W = st.V;
     for (int e = threadIdx.x; e < R2; e += BS) {
         const int u = e / R, v = e % R;
         const double l = lam[v];
-        a.W[u * R + (R - 1 - v)] = l > 1e-8 ? Vs[u * R + v] / sqrt(l) : 0.0;
+        const double w = l > 1e-8 ? vecs[u * R + v] / sqrt(l) : 0.0;
+        W[u * R + (R - 1 - v)] = w;
+        a.W[u * R + (R - 1 - v)] = w;
     }
     __syncthreads();
     // pinv = W W^T (140-150), symmetric storage
@@ -298,11 +415,9 @@ __global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
         const int r = e / R, c = e % R;
         const int lo = r < c ? r : c, hi = r < c ? c : r;
         double acc = 0.0;
-        for (int k = 0; k < R; k++) acc = acc + a.W[lo * R + k] * a.W[hi * R + k];
-        A[e] = acc;
+        for (int k = 0; k < R; k++) acc = acc + W[lo * R + k] * W[hi * R + k];
+        a.pinv[e] = acc;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < R2; e += BS) a.pinv[e] = A[e];
 }
 
 template <int BS>
@@ -311,76 +426,80 @@ __global__ void __launch_bounds__(BS) k_eigen_modes(ComputeMArgs a) {
     extern __shared__ double smem[];
     __shared__ JacobiShared js;
     __shared__ double diag_tmp[128];
+    __shared__ uint8_t pa[8256], pb[8256];
     const int t = blockIdx.x;  // position in the sampled-mode list
     const int k = a.modes[t];
     const int R = a.R, R2 = R * R;
+    const int LD = R + 1;
+    const JacobiStore st = jacobi_store(smem, a.work + (uint64_t)(t + 1) * 3 * LD * R, R);
+    double* A = st.A;
     double* E = a.E + (uint64_t)k * R2;
+    double* El = st.V;  // E staged next to the Jacobi buffers
+    fill_pair_table(pa, pb, R);
     if (t + 1 < a.m) {
-        const bool in_lds = 2 * R2 <= 8192;
-        double* A = in_lds ? smem : a.work + (uint64_t)(t + 1) * 2 * R2;
-        double* V = A + R2;
         double* lam = a.lam_out + (uint64_t)(t + 1) * R;
         // P_k = (G_{modes[t+1]} (.) (... (.) G_{modes[m-1]})) (.) pinv
+        for (int e = threadIdx.x; e < R2; e += BS) A[(e / R) * LD + e % R] = 1.0;
+        for (int s = a.m - 1; s > t; s--) {
+            const double* G = a.roots[a.modes[s]];
+            for (int e = threadIdx.x; e < R2; e += BS) {
+                const int r = e / R, c = e % R;
+                A[r * LD + c] = packed_get(G, R, r, c) * A[r * LD + c];
+            }
+        }
         for (int e = threadIdx.x; e < R2; e += BS) {
             const int r = e / R, c = e % R;
-            double g = 1.0;
-            for (int s = a.m - 1; s > t; s--) g = packed_get(a.roots[a.modes[s]], R, r, c) * g;
-            A[e] = g * a.pinv[e];
+            A[r * LD + c] = A[r * LD + c] * a.pinv[e];
         }
         __syncthreads();
-        // sorted eigenvectors land in A (only its diagonal is still needed
-        // by the sort, and that is copied out before the permutation)
-        jacobi_block<BS>(A, V, R, lam, A, js, diag_tmp);
+        jacobi_block<BS>(A, st.V, R, lam, st.vecs, js, diag_tmp);
+        const double* vecs = st.vecs;
         for (int e = threadIdx.x; e < R2; e += BS) {
             const int r = e / R, u = e % R;
             // sqrt(mu) clamped at 0 (the reference takes sqrt of a negative
             // eigenvalue and produces NaN, efficient_krp_sampler.hpp:176-180)
             const double mu = lam[r];
-            E[r * R + u] = A[u * R + r] * sqrt(mu > 0.0 ? mu : 0.0);
+            const double v = vecs[u * R + r] * sqrt(mu > 0.0 ? mu : 0.0);
+            El[r * R + u] = v;
+            E[r * R + u] = v;
         }
     } else {
         for (int e = threadIdx.x; e < R2; e += BS) {
             const int r = e / R, u = e % R;
-            E[r * R + u] = a.W[u * R + r];
+            const double v = a.W[u * R + r];
+            El[r * R + u] = v;
+            E[r * R + u] = v;
         }
     }
     __syncthreads();
     // Eigen tree over the R rows of E with F = 1, then every node (.) G_k
     // (187-193).  Leaves: node L-1+i holds row leaf_index(node).
     const Shape es = make_shape(R, 1);
-    const int T = (int)tri_size(R);
-    double* tree = a.etree + (uint64_t)k * a.etree_nodes * a.ts64;
-    const uint64_t first_leaf = es.node_count - es.leaf_count;
-    for (uint64_t w = threadIdx.x; w < es.leaf_count * a.ts64; w += BS) {
-        const uint64_t i = w / a.ts64, e = w % a.ts64;
-        const uint64_t node = first_leaf + i;
-        const uint64_t row = leaf_index(es, node);
-        double v = 0.0;
-        if ((int)e < T) {
-            uint32_t ra = 0, off = (uint32_t)e;
-            while (off >= (uint32_t)R - ra) {
-                off -= R - ra;
-                ra++;
-            }
-            v = E[row * R + ra] * E[row * R + ra + off];
-        }
-        tree[node * a.ts64 + e] = v;
+    const uint32_t T = (uint32_t)tri_size(R), ts = a.ts64;
+    double* tree = a.etree + (uint64_t)k * a.etree_nodes * ts;
+    const uint32_t L = (uint32_t)es.leaf_count, first_leaf = (uint32_t)(es.node_count - es.leaf_count);
+    for (uint32_t w = threadIdx.x; w < L * ts; w += BS) {
+        const uint32_t i = w / ts, e = w - i * ts;
+        const uint32_t node = first_leaf + i;
+        const uint32_t row = (uint32_t)leaf_index(es, node);
+        tree[node * ts + e] = e < T ? El[row * R + pa[e]] * El[row * R + pb[e]] : 0.0;
     }
     __syncthreads();
-    int64_t start = (int64_t)es.nodes_before_lfill, end = (int64_t)first_leaf;
-    for (int64_t level = (int64_t)es.lfill_level; level >= 0; level--) {
-        for (uint64_t w = threadIdx.x; w < (uint64_t)(end - start) * a.ts64; w += BS) {
-            const uint64_t c = start + w / a.ts64, e = w % a.ts64;
-            tree[c * a.ts64 + e] = tree[(2 * c + 1) * a.ts64 + e] + tree[(2 * c + 2) * a.ts64 + e];
+    int start = (int)es.nodes_before_lfill, end = (int)first_leaf;
+    for (int level = (int)es.lfill_level; level >= 0; level--) {
+        for (uint32_t w = threadIdx.x; w < (uint32_t)(end - start) * ts; w += BS) {
+            const uint32_t cc = w / ts, e = w - cc * ts;
+            const uint32_t c = start + cc;
+            tree[c * ts + e] = tree[(2 * c + 1) * ts + e] + tree[(2 * c + 2) * ts + e];
         }
         __syncthreads();
         end = start;
         start = ((start + 1) / 2) - 1;
     }
     const double* Gk = a.roots[k];
-    for (uint64_t w = threadIdx.x; w < es.node_count * a.ts64; w += BS) {
-        const uint64_t e = w % a.ts64;
-        if ((int)e < T) tree[w] = tree[w] * Gk[e];
+    for (uint32_t w = threadIdx.x; w < (uint32_t)es.node_count * ts; w += BS) {
+        const uint32_t node = w / ts, e = w - node * ts;
+        if (e < T) tree[w] = tree[w] * Gk[e];
     }
 }
 
@@ -450,47 +569,51 @@ __device__ inline void make_coefs(const double* xs, int R, int g, int epl,
     }
 }
 
-// x^T G x over one packed node.  The loads are unconditional so that all EPL
-// of them are in flight together: an entry index past the packed size T is
-// clamped to T-1 (inside the node) and its coefficient is zero.  (The node
-// stride is only rounded to 16/32 entries, which SG * ceil(T/SG) can exceed.)
-template <typename TN, typename TC, int EPL, int SG>
-__device__ inline double node_qform(const TN* __restrict__ node, const TC (&c)[EPL], int g, int epl, int T) {
+// x^T G x over one packed node.  The EPL loads per lane are unconditional and
+// use one base address with immediate offsets, so all of them are in flight
+// together.  Entries past the packed size T (the node stride is rounded to
+// 16/32 entries, which SG * EPL can exceed) belong to the next node or to the
+// zeroed tail pad every tree allocation carries; their coefficients are zero
+// and the values are finite, so they add nothing.
+// Nodes of FP32 levels are widened to FP64 before the multiply-add, so the
+// only FP32 error is the storage rounding of each entry.
+template <typename TN, int EPL, int SG>
+__device__ inline double node_qform(const TN* __restrict__ node, const double (&c)[EPL], int g) {
     TN v[EPL];
+    const TN* base = node + g;
 #pragma unroll
-    for (int i = 0; i < EPL; i++) {
-        const int e = g + SG * i;
-        v[i] = node[e < T ? e : T - 1];
-    }
-    TC acc0 = 0, acc1 = 0;
+    for (int i = 0; i < EPL; i++) v[i] = base[SG * i];
+    double acc0 = 0, acc1 = 0;
 #pragma unroll
     for (int i = 0; i < EPL; i += 2) {
-        acc0 += c[i] * (TC)v[i];
-        if (i + 1 < EPL) acc1 += c[i + 1] * (TC)v[i + 1];
+        acc0 += c[i] * (double)v[i];
+        if (i + 1 < EPL) acc1 += c[i + 1] * (double)v[i + 1];
     }
-    (void)epl;
-    return group_sum<SG>((double)acc0 + (double)acc1);
+    return group_sum<SG>(acc0 + acc1);
+}
+
+// Quadratic form of the left child `child` at depth `depth`: FP64 node for
+// depth < d64 (or every level in FP64 mode), FP32 node below.
+template <int EPL, int SG, bool USE32>
+__device__ inline double child_qform(const ModeDev& md, const double* tree64, uint64_t child, uint32_t depth,
+                                     uint32_t ts64, uint32_t ts32, const double (&c)[EPL], int g) {
+    if (!USE32 || depth < md.d64) return node_qform<double, EPL, SG>(tree64 + child * ts64, c, g);
+    return node_qform<float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c, g);
 }
 
 // One inverse-CDF descent (partition_tree.hpp:251-332) for the group's sample.
-template <int RMAX, int SG>
+template <int RMAX, int SG, bool USE32>
 __device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
-                                   const double (&c64)[DrawCfg<RMAX, SG>::EPL],
-                                   const float (&c32)[DrawCfg<RMAX, SG>::EPL], bool use32,
-                                   const double* tree64, double u, int g, int epl, int T, double& low,
-                                   double& high) {
+                                   const double (&c64)[DrawCfg<RMAX, SG>::EPL], const double* tree64, double u,
+                                   int g, double& low, double& high) {
     constexpr int EPL = DrawCfg<RMAX, SG>::EPL;
-    const double m = node_qform<double, double, EPL, SG>(tree64, c64, g, epl, T);
+    const double m = node_qform<double, EPL, SG>(tree64, c64, g);
     uint64_t c = 0;
     low = 0.0;
     high = 1.0;
     for (uint32_t d = 0; d < sh.lfill_level; d++) {
         const uint64_t child = 2 * c + 1;
-        double mL;
-        if (!use32 || d + 1 < md.d64)
-            mL = node_qform<double, double, EPL, SG>(tree64 + child * ts64, c64, g, epl, T);
-        else
-            mL = node_qform<float, float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c32, g, epl, T);
+        const double mL = child_qform<EPL, SG, USE32>(md, tree64, child, d + 1, ts64, ts32, c64, g);
         const double cutoff = low + mL / m;
         if (u <= cutoff) {
             c = child;
@@ -504,11 +627,8 @@ __device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t 
         const bool leaf = 2 * c + 1 >= sh.node_count;
         if (!leaf) {
             const uint64_t child = 2 * c + 1;
-            double mL;
-            if (!use32 || sh.lfill_level + 1 < md.d64)
-                mL = node_qform<double, double, EPL, SG>(tree64 + child * ts64, c64, g, epl, T);
-            else
-                mL = node_qform<float, float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c32, g, epl, T);
+            const double mL =
+                child_qform<EPL, SG, USE32>(md, tree64, child, (uint32_t)sh.lfill_level + 1, ts64, ts32, c64, g);
             const double cutoff = low + mL / m;
             if (u <= cutoff) {
                 c = child;
@@ -522,14 +642,15 @@ __device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t 
     return c;
 }
 
-// U_row . x for one leaf row.  With R == RMAX the row is read with 16-byte
-// vector loads (rows are 16-byte aligned then); otherwise entry by entry with
-// the column index clamped (x is zero past R).  All loads are unconditional.
+// U_row . x for one leaf row, accumulated in FP64.  With R == RMAX the row is
+// read with 16-byte vector loads (rows are 16-byte aligned then); otherwise
+// entry by entry with the column index clamped (x is zero past R).  All loads
+// are unconditional.
 template <typename TU, int RMAX>
-__device__ inline TU row_dot(const TU* __restrict__ row, const double* xs, int R) {
+__device__ inline double row_dot(const TU* __restrict__ row, const double* xs, int R) {
     constexpr int W = 16 / sizeof(TU);
     typedef TU V __attribute__((ext_vector_type(W)));
-    TU acc0 = 0, acc1 = 0;
+    double acc0 = 0, acc1 = 0;
     if (R == RMAX) {
         const V* rv = reinterpret_cast<const V*>(row);
         V buf[RMAX / W];
@@ -539,8 +660,8 @@ __device__ inline TU row_dot(const TU* __restrict__ row, const double* xs, int R
         for (int q = 0; q < RMAX / W; q++)
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                if (w & 1) acc1 += buf[q][w] * (TU)xs[q * W + w];
-                else acc0 += buf[q][w] * (TU)xs[q * W + w];
+                if (w & 1) acc1 += (double)buf[q][w] * xs[q * W + w];
+                else acc0 += (double)buf[q][w] * xs[q * W + w];
             }
     } else {
         TU buf[RMAX];
@@ -548,8 +669,8 @@ __device__ inline TU row_dot(const TU* __restrict__ row, const double* xs, int R
         for (int b = 0; b < RMAX; b++) buf[b] = row[b < R ? b : R - 1];
 #pragma unroll
         for (int b = 0; b < RMAX; b++) {
-            if (b & 1) acc1 += buf[b] * (TU)xs[b];
-            else acc0 += buf[b] * (TU)xs[b];
+            if (b & 1) acc1 += (double)buf[b] * xs[b];
+            else acc0 += (double)buf[b] * xs[b];
         }
     }
     return acc0 + acc1;
@@ -568,8 +689,8 @@ __device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint
     for (int k = 0; k < RPL; k++) {
         const uint32_t t = g + SG * k;
         const uint32_t tc = t < rows ? t : rows - 1;
-        const TU d = row_dot<TU, RMAX>(U + (r0 + tc) * (uint64_t)R, xs, R);
-        q2[k] = t < rows ? (double)d * (double)d : 0.0;
+        const double d = row_dot<TU, RMAX>(U + (r0 + tc) * (uint64_t)R, xs, R);
+        q2[k] = t < rows ? d * d : 0.0;
     }
     double incl[RPL];
     double run = 0.0;
@@ -643,14 +764,13 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 if (g + SG * q < R) xs[g + SG * q] = h[q];
             wave_sync_lds();
             double c64[EPL];
-            float c32[EPL];
             make_coefs<RMAX, SG>(xs, R, g, epl, c64);
             double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i]
                                           : philox_uniform(a.seed, a.call, i, 2 * k))
                                : 0.5;
             double low, high;
-            const uint64_t ce = descend<RMAX, SG>(md, a.esh, a.ts64, a.ts32, c64, c32, false, md.etree,
-                                                  ue, g, epl, T, low, high);
+            const uint64_t ce =
+                descend<RMAX, SG, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, g, low, high);
             const uint64_t r = leaf_index(a.esh, ce);
             wave_sync_lds();
             // scaled_h = h (.) E_k[r]
@@ -660,15 +780,10 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
             wave_sync_lds();
             // ---- Gram stage: x = scaled_h -------------------------------------
             make_coefs<RMAX, SG>(xs, R, g, epl, c64);
-            if constexpr (fp32) {
-#pragma unroll
-                for (int q = 0; q < EPL; q++) c32[q] = (float)c64[q];
-            }
             const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                                 : philox_uniform(a.seed, a.call, i, 2 * k + 1))
                                      : 0.5;
-            const uint64_t c = descend<RMAX, SG>(md, md.sh, a.ts64, a.ts32, c64, c32, fp32, md.tree64, ug,
-                                                 g, epl, T, low, high);
+            const uint64_t c = descend<RMAX, SG, fp32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, g, low, high);
             const uint64_t idx = leaf_index(md.sh, c);
             uint32_t res = 0;
             if (md.sh.F > 1) {

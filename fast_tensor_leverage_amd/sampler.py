@@ -168,9 +168,18 @@ class Sampler:
         _lib.check(_lib.load().ftl_sampler_set_seed(self.handle, int(seed) & (2**64 - 1), int(counter)),
                    "set_seed")
 
+    def set_option(self, key: str, value: int):
+        """Tuning knob of the draw kernel (see ftl_sampler_set_option)."""
+        _lib.check(_lib.load().ftl_sampler_set_option(self.handle, key.encode(), int(value)), "set_option")
+
+    def get_option(self, key: str) -> int:
+        out = ctypes.c_int64()
+        _lib.check(_lib.load().ftl_sampler_get_option(self.handle, key.encode(), ctypes.byref(out)), "get_option")
+        return int(out.value)
+
     def set_tuning(self, group_lanes: int = 0, grid_blocks: int = 0):
-        _lib.check(_lib.load().ftl_sampler_set_tuning(self.handle, int(group_lanes), int(grid_blocks)),
-                   "set_tuning")
+        self.set_option("group_lanes", group_lanes)
+        self.set_option("grid_blocks", grid_blocks)
 
     def enable_timing(self, on: bool = True):
         _lib.check(_lib.load().ftl_sampler_enable_timing(self.handle, 1 if on else 0), "enable_timing")

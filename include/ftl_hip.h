@@ -120,9 +120,14 @@ int ftl_sampler_info(ftl_sampler* s, uint64_t* out);
 /* Stream of the handle (hipStream_t) for callers that want to order work. */
 void* ftl_sampler_stream(ftl_sampler* s);
 
-/* Tuning knobs (autotuner surface): group width SG (lanes per sample) of the
- * draw kernel for the handle's R, and waves per SIMD cap.  0 = default. */
-int ftl_sampler_set_tuning(ftl_sampler* s, uint32_t group_lanes, uint32_t grid_blocks);
+/* Tuning knobs (the autotuner's surface; no counterpart in the reference).
+ * Keys: "group_lanes" (lanes per sample in the draw kernel: 8/16/32/64,
+ * 0 = default), "grid_blocks" (cap on draw-kernel blocks, 0 = one block per
+ * 256/group_lanes samples), "fp32_levels" (FP32 mode:
+ * number of bottom tree levels stored FP32; rebuilds the FP32 copies).
+ * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree). */
+int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value);
+int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value);
 
 /* Kernel timing with HIP events recorded on the launch stream.  While enabled,
  * every draw records four events without synchronising; last_timing waits for
