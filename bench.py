@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--I", type=int, default=I_ROWS)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sampler runs on a single MI355X in this build "
@@ -143,7 +146,8 @@ def main():
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         s.enable_timing(False)
-        results[prec] = {"elapsed": elapsed, "timing": timing, "build_s": t_build, "info": info}
+        results[prec] = {"elapsed": elapsed, "timing": timing, "build_s": t_build, "info": info,
+                         "group_lanes": s.get_option("group_lanes") or 32}
         samples_out[prec] = d_samples.cpu().numpy().view(np.uint64).copy()
         log(f"{prec}: {args.steps * J / elapsed:,.0f} samples/s; kernels {timing}; build {t_build:.2f} s")
         del s, d_samples, d_h, d_w
@@ -158,11 +162,13 @@ def main():
     bytes64 = algorithmic_bytes(I, R, J, N, 0, False)
     t_kernel = f32["timing"]["draw_ms"] * 1e-3
     achieved = bytes32 / t_kernel
+    # HBM bytes per launch of the FP32 draw kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)
     traffic = None
-    tfile = os.path.join(HERE, "profiles", "krp_draw_fp32_traffic.json")
+    tfile = os.path.join(HERE, "profiles", "k_krp_draw_N3_I2p23_R32_J50000_traffic.json")
     if os.path.exists(tfile) and I == I_ROWS:
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f)["fp32"]["hbm_bytes_per_launch"]
 
     cpu = None
     extra_cmp = {}
@@ -187,7 +193,9 @@ def main():
                    "N": N, "I": I, "R": R, "J": J, "j": j},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_krp_draw<32,16,256,fp32>", "kernel_ms": f32["timing"]["draw_ms"],
+                     "traffic_GBps": (traffic / t_kernel / 1e9) if traffic else None,
+                     "kernel": f"k_krp_draw<32,{f32['group_lanes']},256,fp32>",
+                     "kernel_ms": f32["timing"]["draw_ms"],
                      "algorithmic_bytes_per_launch": bytes32},
         "cpu_baseline": cpu,
         "fp64_gpu_samples_per_s": v64,
@@ -200,6 +208,12 @@ def main():
     }
     line.update(extra_cmp)
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB device-to-device copy done")
 
 
 if __name__ == "__main__":
