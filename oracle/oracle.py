@@ -24,7 +24,19 @@ built here: it needs cblas.h/lapacke.h):
   sampler_draw            exact leverage scores by QR of the materialised KRP:
                           TV distance within the multinomial noise band, and
                           per-draw weights = R / (J * h^T pinv(G) h) in NumPy
-  als (ftl_oracle_als)    see the header of ftl_oracle_als.cpp
+  sparse mttkrp           brute-force scan of every nonzero: matched nonzero
+                          ids equal as integers, result bitwise equal
+                          (tests/test_oracle_als.py)
+  residual / lowrank      dense reconstruction [[sigma; U]] on small tensors
+  normsq
+  compute_pinv(_square)   numpy pinv; explicit eigenbasis with the 1e-11 cut
+                          and the target-rank truncation
+  renormalize_columns     numpy column norms with the 1e-7 floor
+  als_dedup               numpy.unique(axis=0) with first index and counts
+  als_update              (1) exact recovery of a noise-free low-rank factor,
+                          (2) the weighted sampled least-squares solution
+                          computed densely from the same samples, h, weights,
+                          (3) sweeps drive the fit of a noisy tensor to ~1
 """
 from __future__ import annotations
 
@@ -90,12 +102,15 @@ def _als_bind(L):
     L.ftlo_sparse_destroy.argtypes = [_vp]
     L.ftlo_sparse_normsq.argtypes = [_vp]
     L.ftlo_sparse_normsq.restype = ctypes.c_double
-    L.ftlo_sparse_mttkrp.argtypes = [_vp, _u64, _vp, _u64, _vp, _u64, _u64, _vp, _vp]
+    L.ftlo_sparse_mttkrp.argtypes = [_vp, _u64, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _u64]
     L.ftlo_sparse_mttkrp.restype = _u64
-    L.ftlo_sparse_residual_normsq.argtypes = [_vp, _vp, _vp, _u64]
+    L.ftlo_sparse_residual_normsq.argtypes = [_vp, _vp, _vp, _vp, _u64]
     L.ftlo_sparse_residual_normsq.restype = ctypes.c_double
+    L.ftlo_lowrank_normsq.argtypes = [_vp, _vp, _u64, _u64, _vp]
+    L.ftlo_lowrank_normsq.restype = ctypes.c_double
     L.ftlo_renormalize_columns.argtypes = [_vp, _u64, _u64, _vp]
-    L.ftlo_als_update.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp, _vp, _vp]
+    L.ftlo_als_update.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp, ctypes.c_int, ctypes.c_int,
+                                  _vp, _vp]
     L.ftlo_als_update.restype = ctypes.c_int
 
 
@@ -232,3 +247,138 @@ class OracleSampler:
         if want_eig:
             out["eig"] = eig
         return out
+
+
+# ----------------------------------------------------------------------------
+# sampled CP-ALS level (SparseTensor / LowRankTensor / ALS surface)
+# ----------------------------------------------------------------------------
+def _ptrs(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class OracleSparse:
+    """SparseTensor(idx, values, "sort") restated (sparse_tensor.hpp:46-119,
+    sort_lookup.hpp:33-197).  idx is nnz x N uint32, values nnz float64."""
+
+    def __init__(self, idx, values):
+        self.idx = _c(idx, np.uint32)
+        self.values = _c(values, np.float64)
+        if self.idx.ndim != 2 or self.values.shape != (self.idx.shape[0],):
+            raise ValueError("idx must be nnz x N and values nnz")
+        self.nnz, self.N = self.idx.shape
+        self.dims = [int(self.idx[:, k].max()) + 1 if self.nnz else 1 for k in range(self.N)]
+        self._h = lib().ftlo_sparse_create(_p(self.idx), _p(self.values), self.nnz, self.N)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ftlo_sparse_destroy(h)
+            self._h = None
+
+    def normsq(self) -> float:
+        return float(lib().ftlo_sparse_normsq(self._h))
+
+    def mttkrp(self, samples, lhs, j, Ij, want_matches=False):
+        """execute_downsampled_mttkrp: samples Jp x N, lhs Jp x R -> Ij x R."""
+        samples = _c(samples, np.uint64)
+        lhs = _c(lhs, np.float64)
+        Jp, R = lhs.shape
+        if samples.shape != (Jp, self.N):
+            raise ValueError("samples must be Jp x N")
+        out = np.empty((Ij, R))
+        counts = np.zeros(Jp, dtype=np.uint64) if want_matches else None
+        cap = 0
+        ids = None
+        if want_matches:
+            cap = int(lib().ftlo_sparse_mttkrp(self._h, j, _p(samples), Jp, _p(lhs), R, Ij, _p(out),
+                                               None, None, 0))
+            ids = np.zeros(max(cap, 1), dtype=np.uint64)
+        found = int(lib().ftlo_sparse_mttkrp(self._h, j, _p(samples), Jp, _p(lhs), R, Ij, _p(out),
+                                             _p(counts), _p(ids), cap))
+        if want_matches:
+            return out, found, counts, ids[:found]
+        return out, found
+
+    def residual_normsq(self, sigma, U_list) -> float:
+        U = [_c(u, np.float64) for u in U_list]
+        sigma = _c(sigma, np.float64)
+        dims = np.array([u.shape[0] for u in U], dtype=np.uint64)
+        return float(lib().ftlo_sparse_residual_normsq(self._h, _p(sigma), _ptrs(U), _p(dims), U[0].shape[1]))
+
+
+def lowrank_normsq(U_list, sigma) -> float:
+    U = [_c(u, np.float64) for u in U_list]
+    sigma = _c(sigma, np.float64)
+    dims = np.array([u.shape[0] for u in U], dtype=np.uint64)
+    return float(lib().ftlo_lowrank_normsq(_ptrs(U), _p(dims), len(U), U[0].shape[1], _p(sigma)))
+
+
+def compute_pinv(A):
+    """compute_pinv (common.h:271-323): pinv(A^T A), target rank min(I, R)."""
+    A = _c(A, np.float64)
+    I, R = A.shape
+    out = np.empty((R, R))
+    lib().ftlo_compute_pinv(_p(A), I, R, _p(out))
+    return out
+
+
+def compute_pinv_square(M, target_rank):
+    M = _c(M, np.float64)
+    R = M.shape[0]
+    out = np.empty((R, R))
+    lib().ftlo_compute_pinv_square(_p(M), R, target_rank, _p(out))
+    return out
+
+
+def renormalize_columns(U):
+    """In place; returns the column norms (renormalize_columns(j))."""
+    assert U.dtype == np.float64 and U.flags.c_contiguous
+    norms = np.empty(U.shape[1])
+    lib().ftlo_renormalize_columns(_p(U), U.shape[0], U.shape[1], _p(norms))
+    return norms
+
+
+def als_dedup(samples, j):
+    """Sort/unique of execute_ds_als_update: samples N x J -> (first, count,
+    tuples J' x N with column j zero) in ascending lexicographic order."""
+    samples = _c(samples, np.uint64)
+    N, J = samples.shape
+    first = np.empty(J, dtype=np.uint64)
+    count = np.empty(J, dtype=np.uint64)
+    tuples = np.empty((J, N), dtype=np.uint64)
+    u = int(lib().ftlo_als_dedup(_p(samples), N, J, j, _p(first), _p(count), _p(tuples), None, None, None))
+    return first[:u], count[:u], tuples[:u]
+
+
+class OracleALS:
+    """ALS(LowRankTensor(R, U), SparseTensor) + initialize_ds_als(J,
+    "efficient") restated (als.hpp:31-220).  The factors live in
+    ``self.U`` (updated in place), ``self.sigma`` holds the column norms of
+    the last renormalized mode."""
+
+    def __init__(self, sparse: OracleSparse, U_list, J: int, sigma=None):
+        R = U_list[0].shape[1]
+        self.X = sparse
+        self.sampler = OracleSampler(U_list, J, R)
+        self.U = self.sampler.U
+        self.N, self.J, self.R = len(self.U), int(J), R
+        self.sigma = np.ones(R) if sigma is None else _c(sigma, np.float64).copy()
+
+    def update(self, j, draws, renormalize=True, update_sampler=True):
+        draws = _c(draws, np.float64)
+        if draws.size != 2 * self.N * self.J:
+            raise ValueError("draws must hold 2*N*J uniforms")
+        dims = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
+        diag = np.zeros(4)
+        rc = lib().ftlo_als_update(self.sampler._h, self.X._h, _ptrs(self.U), _p(dims), self.N, self.J, self.R, j,
+                                   _p(draws), int(renormalize), int(update_sampler), _p(self.sigma), _p(diag))
+        if rc != 0:
+            raise RuntimeError("oracle ALS update failed")
+        return {"unique_samples": int(diag[0]), "found": int(diag[1])}
+
+    def residual_normsq(self) -> float:
+        return self.X.residual_normsq(self.sigma, self.U)
+
+    def fit(self) -> float:
+        """compute_exact_fit (tensors.py:41-53): 1 - sqrt(res) / ||X||."""
+        return 1.0 - np.sqrt(self.residual_normsq()) / np.sqrt(self.X.normsq())

@@ -94,3 +94,60 @@ def philox_draws(seed, call, N, J):
         out[k * J:(k + 1) * J] = philox_uniforms(seed, call, J, 2 * k)
         out[N * J + k * J:N * J + (k + 1) * J] = philox_uniforms(seed, call, J, 2 * k + 1)
     return out
+
+
+# ---------------------------------------------------------------------------
+# sampled CP-ALS closed forms
+# ---------------------------------------------------------------------------
+def cp_dense(U, sigma=None):
+    """Dense [[sigma; U_0, ..., U_{N-1}]]."""
+    R = U[0].shape[1]
+    sigma = np.ones(R) if sigma is None else sigma
+    out = krp(U) @ sigma
+    return out.reshape([M.shape[0] for M in U])
+
+
+def random_sparse(dims, nnz, rng, unique=True):
+    """nnz x N uint32 coordinates (unique unless asked otherwise) and values."""
+    idx = np.stack([rng.integers(0, d, size=4 * nnz) for d in dims], axis=1).astype(np.uint32)
+    if unique:
+        idx = np.unique(idx, axis=0)
+        idx = idx[rng.permutation(idx.shape[0])[:nnz]]
+    else:
+        idx = idx[:nnz]
+    return np.ascontiguousarray(idx), rng.standard_normal(idx.shape[0])
+
+
+def dense_from_sparse(idx, vals, dims):
+    X = np.zeros(dims)
+    np.add.at(X, tuple(idx[:, k].astype(np.int64) for k in range(idx.shape[1])), vals)
+    return X
+
+
+def brute_mttkrp(idx, vals, samples, lhs, j, Ij):
+    """execute_downsampled_mttkrp by brute force: for each sample (in order)
+    every nonzero (ascending position) equal on all modes but j."""
+    N = idx.shape[1]
+    other = [k for k in range(N) if k != j]
+    out = np.zeros((Ij, lhs.shape[1]))
+    ids, counts = [], []
+    for s in range(samples.shape[0]):
+        hit = np.nonzero(np.all(idx[:, other] == samples[s, other].astype(np.uint32), axis=1))[0]
+        counts.append(len(hit))
+        for z in hit:
+            out[idx[z, j]] += lhs[s] * vals[z]
+            ids.append(z)
+    return out, np.array(counts, dtype=np.uint64), np.array(ids, dtype=np.uint64)
+
+
+def sampled_ls_solution(X, U, j, samples, h, w):
+    """U_j = (sum_s w_s X[.., :, ..](tuple_s) h_s^T) pinv(sum_s w_s h_s h_s^T)
+    over the J drawn samples (duplicates included), dense X."""
+    N = X.ndim
+    J = samples.shape[1]
+    Xj = np.moveaxis(X, j, 0)
+    other = [k for k in range(N) if k != j]
+    cols = Xj[(slice(None),) + tuple(samples[k].astype(np.int64) for k in other)]  # I_j x J
+    mt = (cols * w[None, :]) @ h
+    G = (h * w[:, None]).T @ h
+    return mt @ np.linalg.pinv(G, rcond=1e-13, hermitian=True)
