@@ -6,7 +6,10 @@ The compute path is libftl_hip.so (hand-written HIP kernels, C ABI in
 include/ftl_hip.h); this package is its host-side mirror of the reference's
 Python-visible classes.
 """
+from .als import ALS, LowRankTensor, SparseTensor  # noqa: F401
+from .als_driver import als_prod  # noqa: F401
 from .sampler import Sampler  # noqa: F401
+from .tensors import PyLowRank, PySparseTensor  # noqa: F401
 
-__all__ = ["Sampler"]
+__all__ = ["Sampler", "ALS", "LowRankTensor", "SparseTensor", "PyLowRank", "PySparseTensor", "als_prod"]
 __version__ = "0.1.0"

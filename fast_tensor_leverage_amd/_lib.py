@@ -39,6 +39,33 @@ SIGNATURES = {
     "ftl_sampler_get_option": ([_vp, ctypes.c_char_p, _vp], _i32),
     "ftl_sampler_enable_timing": ([_vp, _i32], _i32),
     "ftl_sampler_last_timing": ([_vp, _vp], _i32),
+    # sparse tensor and sampled CP-ALS
+    "ftl_sparse_create": ([_u32, _u64, _vp, _vp, _vp, _vp], _i32),
+    "ftl_sparse_create_device": ([_u32, _u64, _vp, _vp, _vp, _vp], _i32),
+    "ftl_sparse_destroy": ([_vp], _i32),
+    "ftl_sparse_info": ([_vp, _vp], _i32),
+    "ftl_sparse_normsq": ([_vp, _vp], _i32),
+    "ftl_sparse_mttkrp": ([_vp, _u32, _u64, _vp, _vp, _u64, _vp, _u64, _vp], _i32),
+    "ftl_sparse_lookup": ([_vp, _u32, _u64, _vp, _vp, _vp, _u64, _vp], _i32),
+    "ftl_sparse_residual_normsq": ([_vp, _u64, _vp, _vp, _vp, _vp], _i32),
+    "ftl_als_create": ([_vp, _u32, _vp, _u64, _vp, _vp], _i32),
+    "ftl_als_destroy": ([_vp], _i32),
+    "ftl_als_initialize_ds": ([_vp, _u64, _i32], _i32),
+    "ftl_als_set_seed": ([_vp, _u64, _u64], _i32),
+    "ftl_als_ds_update": ([_vp, _u32, _i32, _i32, _vp], _i32),
+    "ftl_als_ds_update_device": ([_vp, _u32, _i32, _i32, _vp], _i32),
+    "ftl_als_get_factor": ([_vp, _u32, _vp], _i32),
+    "ftl_als_set_factor": ([_vp, _u32, _vp], _i32),
+    "ftl_als_update_sampler": ([_vp, _u32], _i32),
+    "ftl_als_get_sigma": ([_vp, _vp], _i32),
+    "ftl_als_set_sigma": ([_vp, _vp], _i32),
+    "ftl_als_renormalize_columns": ([_vp, _i32], _i32),
+    "ftl_als_residual_normsq": ([_vp, _vp], _i32),
+    "ftl_als_lowrank_normsq": ([_vp, _vp], _i32),
+    "ftl_als_factor_device": ([_vp, _u32, _vp], _i32),
+    "ftl_als_last_stats": ([_vp, _vp], _i32),
+    "ftl_als_enable_timing": ([_vp, _i32], _i32),
+    "ftl_als_stream": ([_vp], _vp),
 }
 
 _lib = None
@@ -61,11 +88,6 @@ def load():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
-    try:
-        from . import _als_sigs
-        _als_sigs.bind(L)
-    except ImportError:
-        pass
     _lib = L
     return L
 

@@ -1,0 +1,970 @@
+// als.hip -- host side of the device sparse tensor and the sampled CP-ALS
+// step: sorted lookups, the ALS update pipeline, renormalization and the
+// exact fit, and their C ABI (include/ftl_hip.h).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "als_kernels.hpp"
+#include "ftl_common.hpp"
+#include "sampler_internal.hpp"
+
+namespace ftl {
+
+namespace {
+
+inline unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap = 65536) {
+    uint64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+inline uint32_t bit_width(uint64_t x) { return x ? 64u - (uint32_t)__builtin_clzll(x) : 0u; }
+
+// Key layout leaving out mode `skip`; fields hold values < 2^bits[k].
+KeyLayout make_layout(uint32_t N, uint32_t skip, const uint32_t* bits) {
+    KeyLayout L{};
+    L.N = N;
+    L.skip = skip;
+    uint32_t sh = 0;
+    for (int k = (int)N - 1; k >= 0; k--) {
+        if ((uint32_t)k == skip) continue;
+        L.shift[k] = sh;
+        L.bits[k] = bits[k];
+        sh += bits[k];
+    }
+    L.total_bits = sh;
+    return L;
+}
+
+// Device allocations of one handle, freed together.
+struct Arena {
+    std::vector<std::pair<void*, size_t>> blocks;
+    size_t bytes = 0;
+
+    template <typename T>
+    T* alloc(uint64_t count) {
+        void* p = nullptr;
+        const size_t b = (size_t)count * sizeof(T);
+        FTL_HIP_CHECK(hipMalloc(&p, b ? b : 16));
+        blocks.push_back({p, b});
+        bytes += b;
+        return (T*)p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        for (size_t i = 0; i < blocks.size(); i++)
+            if (blocks[i].first == p) {
+                bytes -= blocks[i].second;
+                blocks.erase(blocks.begin() + (long)i);
+                break;
+            }
+        (void)hipFree(p);
+    }
+    ~Arena() {
+        for (auto& b : blocks) (void)hipFree(b.first);
+    }
+};
+
+// Waits for a stream when it goes out of scope: declared after an Arena whose
+// buffers the stream's queued work uses, it keeps an early exit (exception)
+// from freeing memory that kernels still read.
+struct StreamDrain {
+    hipStream_t st;
+    ~StreamDrain() {
+        if (st) (void)hipStreamSynchronize(st);
+    }
+};
+
+// Growable temporary storage of the hipCUB primitives.
+struct CubTemp {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* ensure(size_t b) {
+        if (b > bytes) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            FTL_HIP_CHECK(hipMalloc(&p, b));
+            bytes = b;
+        }
+        return p;
+    }
+    ~CubTemp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
+template <typename K, typename V>
+void sort_pairs(CubTemp& t, const K* kin, K* kout, const V* vin, V* vout, uint64_t n, uint32_t bits,
+                hipStream_t st) {
+    const int end_bit = (int)std::max<uint32_t>(1, std::min<uint32_t>(bits, 8 * sizeof(K)));
+    size_t b = 0;
+    FTL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, kin, kout, vin, vout, n, 0, end_bit, st));
+    void* p = t.ensure(b);
+    FTL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(p, b, kin, kout, vin, vout, n, 0, end_bit, st));
+}
+
+void inclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+    size_t b = 0;
+    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b, in, out, n, st));
+    void* p = t.ensure(b);
+    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(p, b, in, out, n, st));
+}
+
+void exclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+    size_t b = 0;
+    FTL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
+    void* p = t.ensure(b);
+    FTL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p, b, in, out, n, st));
+}
+
+#define FTL_LAUNCH_CHECK() FTL_HIP_CHECK(hipGetLastError())
+
+// ids[m] = perm[lo[u] + m - off[u]] for the match numbering of k_expand.
+__global__ void k_match_ids(const uint32_t* __restrict__ off, const uint32_t* __restrict__ lo, uint64_t n, uint64_t T,
+                            const uint32_t* __restrict__ perm, uint64_t* __restrict__ ids) {
+    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < T; m += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t l = 0, h = n;
+        while (h - l > 1) {
+            const uint64_t mid = (l + h) >> 1;
+            if (off[mid] <= m) l = mid;
+            else h = mid;
+        }
+        ids[m] = perm[(uint64_t)lo[l] + (m - off[l])];
+    }
+}
+
+// samples (n x N uint64, row-major) -> uint32 tuples (the reference's cast,
+// sparse_tensor.hpp:95-115)
+__global__ void k_tuples_u32(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
+        out[e] = (uint32_t)in[e];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// matched-nonzero pipeline shared by the plain MTTKRP and the ALS update
+// ---------------------------------------------------------------------------
+struct Matches {
+    uint64_t cap_n = 0, cap_t = 0, cap_r = 0;
+    uint32_t *lo = nullptr, *cnt = nullptr, *off = nullptr;
+    uint32_t *mrow = nullptr, *srow = nullptr, *flags = nullptr, *segid = nullptr;
+    uint32_t *seg_start = nullptr, *seg_row = nullptr, *npieces = nullptr, *piece_off = nullptr;
+    uint64_t *mpay = nullptr, *spay = nullptr;
+    double* partial = nullptr;
+    uint32_t* counters = nullptr;       // device, kCntSlots
+    uint32_t* host_counters = nullptr;  // pinned host mirror
+
+    void ensure_n(Arena& ar, uint64_t n) {
+        if (n <= cap_n) return;
+        ar.release(lo);
+        ar.release(cnt);
+        ar.release(off);
+        lo = ar.alloc<uint32_t>(n);
+        cnt = ar.alloc<uint32_t>(n);
+        off = ar.alloc<uint32_t>(n);
+        cap_n = n;
+    }
+
+    void ensure_t(Arena& ar, uint64_t T, uint64_t R) {
+        if (T <= cap_t && R <= cap_r) return;
+        uint64_t c = std::max<uint64_t>(T, cap_t + cap_t / 2);
+        for (void* p : {(void*)mrow, (void*)srow, (void*)flags, (void*)segid, (void*)seg_start, (void*)seg_row,
+                        (void*)npieces, (void*)piece_off, (void*)mpay, (void*)spay, (void*)partial})
+            ar.release(p);
+        mrow = ar.alloc<uint32_t>(c);
+        srow = ar.alloc<uint32_t>(c);
+        flags = ar.alloc<uint32_t>(c);
+        segid = ar.alloc<uint32_t>(c);
+        seg_start = ar.alloc<uint32_t>(c + 1);
+        seg_row = ar.alloc<uint32_t>(c);
+        npieces = ar.alloc<uint32_t>(c);
+        piece_off = ar.alloc<uint32_t>(c);
+        mpay = ar.alloc<uint64_t>(c);
+        spay = ar.alloc<uint64_t>(c);
+        // pieces W <= D + T / kSegPiece <= T + T / kSegPiece + 1
+        cap_r = std::max<uint64_t>(R, cap_r);
+        partial = ar.alloc<double>((c + c / kSegPiece + 1) * cap_r);
+        cap_t = c;
+    }
+
+    void init(Arena& ar) {
+        counters = ar.alloc<uint32_t>(kCntSlots);
+        FTL_HIP_CHECK(hipHostMalloc((void**)&host_counters, kCntSlots * sizeof(uint32_t), hipHostMallocDefault));
+        for (int i = 0; i < kCntSlots; i++) host_counters[i] = 0;
+    }
+
+    ~Matches() {
+        if (host_counters) (void)hipHostFree(host_counters);
+    }
+};
+
+struct SparseDev {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0;
+    uint64_t nnz = 0;
+    uint64_t dims[kMaxModes] = {};
+    uint32_t bits[kMaxModes] = {};
+    KeyLayout layout[kMaxModes] = {};
+    uint64_t* keys[kMaxModes] = {};
+    uint32_t* perm[kMaxModes] = {};
+    uint32_t* row[kMaxModes] = {};
+    double* val[kMaxModes] = {};
+    double normsq = 0.0;
+    Arena mem;
+    CubTemp cub;
+    Matches mt;
+    uint32_t* tup = nullptr;
+    uint64_t tup_cap = 0;
+
+    ~SparseDev() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void build(uint32_t N_, uint64_t nnz_, const uint32_t* idx, const double* values, const uint64_t* dims_in,
+               bool on_device) {
+        FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
+        FTL_REQUIRE(nnz_ >= 1 && nnz_ < (1ull << 32), "nnz must be in [1, 2^32)");
+        FTL_REQUIRE(idx && values, "null index or value array");
+        N = N_;
+        nnz = nnz_;
+        FTL_HIP_CHECK(hipGetDevice(&device));
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        mt.init(mem);
+        // staging copies of the input (freed at the end of construction)
+        uint32_t* d_idx = nullptr;
+        double* d_val = nullptr;
+        Arena stage;
+        StreamDrain drain{stream};
+        if (on_device) {
+            d_idx = const_cast<uint32_t*>(idx);
+            d_val = const_cast<double*>(values);
+        } else {
+            d_idx = stage.alloc<uint32_t>(nnz * N);
+            d_val = stage.alloc<double>(nnz);
+            FTL_HIP_CHECK(hipMemcpyAsync(d_idx, idx, nnz * N * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+            FTL_HIP_CHECK(hipMemcpyAsync(d_val, values, nnz * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        // mode sizes: given, or max coordinate + 1
+        uint32_t* d_max = stage.alloc<uint32_t>(kMaxModes);
+        FTL_HIP_CHECK(hipMemsetAsync(d_max, 0, kMaxModes * sizeof(uint32_t), stream));
+        hipLaunchKernelGGL(k_coord_max, dim3(grid_for(nnz, 256, 2048)), dim3(256), 0, stream, d_idx, nnz, N, d_max);
+        FTL_LAUNCH_CHECK();
+        uint32_t hmax[kMaxModes];
+        FTL_HIP_CHECK(hipMemcpyAsync(hmax, d_max, sizeof(hmax), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        for (uint32_t k = 0; k < N; k++) {
+            if (dims_in) {
+                FTL_REQUIRE(dims_in[k] >= (uint64_t)hmax[k] + 1,
+                            "a coordinate of mode " + std::to_string(k) + " is >= its given size");
+                dims[k] = dims_in[k];
+            } else {
+                dims[k] = (uint64_t)hmax[k] + 1;
+            }
+            bits[k] = bit_width(dims[k] - 1);
+        }
+        for (uint32_t j = 0; j < N; j++) {
+            layout[j] = make_layout(N, j, bits);
+            if (layout[j].total_bits > 64)
+                throw Error{FTL_EUNSUPPORTED, "coordinates of the modes other than " + std::to_string(j) +
+                                                  " need more than 64 key bits"};
+        }
+        // per mode: stable sort of the nonzeros by their key, then the
+        // mode-j coordinate and the value in that order
+        uint64_t* kin = stage.alloc<uint64_t>(nnz);
+        uint32_t* pin = stage.alloc<uint32_t>(nnz);
+        const unsigned g = grid_for(nnz, 256, 8192);
+        for (uint32_t j = 0; j < N; j++) {
+            keys[j] = mem.alloc<uint64_t>(nnz);
+            perm[j] = mem.alloc<uint32_t>(nnz);
+            row[j] = mem.alloc<uint32_t>(nnz);
+            val[j] = mem.alloc<double>(nnz);
+            hipLaunchKernelGGL(k_nz_keys, dim3(g), dim3(256), 0, stream, d_idx, nnz, layout[j], kin, pin);
+            FTL_LAUNCH_CHECK();
+            sort_pairs(cub, kin, keys[j], pin, perm[j], nnz, layout[j].total_bits, stream);
+            hipLaunchKernelGGL(k_nz_gather, dim3(g), dim3(256), 0, stream, d_idx, d_val, perm[j], nnz, N, j, row[j],
+                               val[j]);
+            FTL_LAUNCH_CHECK();
+        }
+        // ||X||^2
+        double* part = stage.alloc<double>(1024 + 1);
+        hipLaunchKernelGGL(k_sumsq_partial<256>, dim3(1024), dim3(256), 0, stream, d_val, nnz, part);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, stream, part, 1024u, part + 1024);
+        FTL_LAUNCH_CHECK();
+        FTL_HIP_CHECK(hipMemcpyAsync(&normsq, part + 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        // stage's buffers are freed here (after the stream drained)
+    }
+
+    uint32_t* tuples(uint64_t n) {
+        if (n > tup_cap) {
+            mem.release(tup);
+            tup = mem.alloc<uint32_t>(n * N);
+            tup_cap = n;
+        }
+        return tup;
+    }
+
+    ProbeArgs probe_args(uint32_t j, const uint32_t* t, const uint32_t* n_dev, uint64_t n_host, uint64_t n_cap,
+                         Matches& m) const {
+        ProbeArgs a{};
+        a.tup = t;
+        a.n_dev = n_dev;
+        a.n_host = n_host;
+        a.n_cap = n_cap;
+        a.keys = keys[j];
+        a.nnz = nnz;
+        a.L = layout[j];
+        for (uint32_t k = 0; k < N; k++) a.dims[k] = dims[k];
+        a.lo = m.lo;
+        a.cnt = m.cnt;
+        return a;
+    }
+};
+
+// Probe n tuples (n from n_dev when given) on mode j's lookup: per-sample
+// ranges, their exclusive scan and the match total in counters[kCntMatches].
+void probe_and_count(SparseDev& X, uint32_t j, const uint32_t* tup, const uint32_t* n_dev, uint64_t n_host,
+                     uint64_t n_cap, Matches& m, Arena& ar, CubTemp& cub, hipStream_t st) {
+    m.ensure_n(ar, n_cap);
+    ProbeArgs pa = X.probe_args(j, tup, n_dev, n_host, n_cap, m);
+    hipLaunchKernelGGL(k_probe, dim3(grid_for(n_cap, 256)), dim3(256), 0, st, pa);
+    FTL_LAUNCH_CHECK();
+    exclusive_sum(cub, m.cnt, m.off, n_cap, st);
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.off, m.cnt, n_cap, m.counters, (int)kCntMatches);
+    FTL_LAUNCH_CHECK();
+}
+
+template <int G, int CPL>
+void launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
+                   const double* pinv, double* out, hipStream_t st) {
+    constexpr int GPB = 256 / G;
+    const unsigned blocks = grid_for(T, GPB, 16384);
+    hipLaunchKernelGGL((k_item_partial<G, CPL>), dim3(blocks), dim3(256), 0, st, m.seg_start, m.piece_off,
+                       m.counters, m.spay, X.val[j], lhs, R, m.partial);
+    FTL_LAUNCH_CHECK();
+    if (pinv)
+        hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(blocks), dim3(256), 0, st, m.seg_row, m.piece_off,
+                           m.counters, m.partial, R, pinv, out);
+    else
+        hipLaunchKernelGGL((k_seg_final<G, CPL, false>), dim3(blocks), dim3(256), 0, st, m.seg_row, m.piece_off,
+                           m.counters, m.partial, R, pinv, out);
+    FTL_LAUNCH_CHECK();
+}
+
+// Given probed ranges of n samples and T >= 1 matches: list the matches,
+// sort them by output row (stable), form the row segments and their pieces,
+// reduce, and write each touched row of out (times pinv when given).
+void reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
+                    const double* pinv, double* out, Matches& m, Arena& ar, CubTemp& cub, hipStream_t st) {
+    m.ensure_t(ar, T, R);
+    hipLaunchKernelGGL(k_expand, dim3(grid_for(T, 256)), dim3(256), 0, st, m.off, m.lo, n, T, X.row[j], m.mrow,
+                       m.mpay);
+    FTL_LAUNCH_CHECK();
+    sort_pairs(cub, m.mrow, m.srow, m.mpay, m.spay, T, X.bits[j], st);
+    hipLaunchKernelGGL(k_head_flags<uint32_t>, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, T, m.flags);
+    FTL_LAUNCH_CHECK();
+    inclusive_sum(cub, m.flags, m.segid, T, st);
+    hipLaunchKernelGGL(k_seg_heads, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, m.segid, T, m.seg_start,
+                       m.seg_row, m.counters);
+    FTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_pieces, dim3(grid_for(T, 256)), dim3(256), 0, st, m.seg_start, m.counters, T,
+                       m.npieces);
+    FTL_LAUNCH_CHECK();
+    exclusive_sum(cub, m.npieces, m.piece_off, T, st);
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, T, m.counters,
+                       (int)kCntPieces);
+    FTL_LAUNCH_CHECK();
+    if (R <= 32) launch_reduce<32, 1>(X, j, m, T, lhs, R, pinv, out, st);
+    else if (R <= 64) launch_reduce<64, 1>(X, j, m, T, lhs, R, pinv, out, st);
+    else launch_reduce<64, 2>(X, j, m, T, lhs, R, pinv, out, st);
+}
+
+// ---------------------------------------------------------------------------
+// LowRankTensor + ALS (als.hpp:31-220, low_rank_tensor.hpp)
+// ---------------------------------------------------------------------------
+struct Als {
+    SparseDev* X = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0;
+    uint64_t R = 0;
+    uint64_t I[kMaxModes] = {};
+    uint32_t ibits[kMaxModes] = {};
+    double* U[kMaxModes] = {};
+    double *sigma = nullptr, *norms = nullptr, *colpart = nullptr, *scal = nullptr, *resid_part = nullptr,
+           *gram_parts = nullptr;
+    Arena mem;
+    CubTemp cub;
+    Matches mt;
+    // sampled update state (initialize_ds_als)
+    Sampler* sampler = nullptr;
+    uint64_t J = 0;
+    KeyLayout dlayout[kMaxModes] = {};
+    uint64_t* samples = nullptr;
+    double *h = nullptr, *w = nullptr, *draws_stage = nullptr;
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    uint32_t *vals_a = nullptr, *vals_b = nullptr, *flags = nullptr, *uid = nullptr;
+    uint32_t *utup = nullptr, *ufirst = nullptr, *ustart = nullptr;
+    double *hs = nullptr, *hdd = nullptr, *gpart = nullptr, *pinv = nullptr, *work = nullptr, *lam = nullptr;
+    // diagnostics
+    bool timing = false;
+    std::array<hipEvent_t, 5> ev{};
+    bool have_events = false;
+    uint64_t last_unique = 0, last_found = 0;
+
+    ~Als() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (sampler) sampler_delete(sampler);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void init(SparseDev* X_, uint32_t N_, const uint64_t* I_, uint64_t R_, const double* const* U_host) {
+        FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
+        FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
+        FTL_REQUIRE(I_ && U_host, "null factor list");
+        X = X_;
+        N = N_;
+        R = R_;
+        if (X) FTL_REQUIRE(X->N == N, "the factor count differs from the sparse tensor's mode count");
+        FTL_HIP_CHECK(hipGetDevice(&device));
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        static bool attrs_set = false;
+        if (!attrs_set) {
+            // A ((R+1) R doubles) in LDS, plus V and the eigenvectors when R <= 64
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<kJacobiThreads>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
+            attrs_set = true;
+        }
+        mt.init(mem);
+        for (uint32_t k = 0; k < N; k++) {
+            FTL_REQUIRE(I_[k] >= 1 && I_[k] < (1ull << 32), "factor row counts must be in [1, 2^32)");
+            FTL_REQUIRE(U_host[k], "null factor matrix");
+            if (X)
+                FTL_REQUIRE(I_[k] >= X->dims[k], "factor " + std::to_string(k) +
+                                                     " has fewer rows than the sparse tensor's mode size");
+            I[k] = I_[k];
+            ibits[k] = bit_width(I[k] - 1);
+            U[k] = mem.alloc<double>(I[k] * R);
+            FTL_HIP_CHECK(hipMemcpyAsync(U[k], U_host[k], I[k] * R * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        const uint64_t T = R * (R + 1) / 2;
+        sigma = mem.alloc<double>(R);
+        norms = mem.alloc<double>(R);
+        colpart = mem.alloc<double>((uint64_t)kColBlocks * R);
+        scal = mem.alloc<double>(4);
+        resid_part = mem.alloc<double>(kResidBlocks);
+        gram_parts = mem.alloc<double>((uint64_t)N * kGramBlocks * T);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(128), 0, stream, sigma, R, 1.0);
+        FTL_LAUNCH_CHECK();
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void initialize_ds(uint64_t J_, int precision) {
+        FTL_REQUIRE(X, "the sampled update needs a sparse tensor");
+        FTL_REQUIRE(N >= 2, "the sampled update needs at least two modes");
+        FTL_REQUIRE(J_ >= 1 && J_ < (1ull << 31), "J must be in [1, 2^31)");
+        FTL_REQUIRE(!sampler, "initialize_ds_als was already called");
+        for (uint32_t j = 0; j < N; j++) {
+            dlayout[j] = make_layout(N, j, ibits);
+            if (dlayout[j].total_bits > 64)
+                throw Error{FTL_EUNSUPPORTED, "sample coordinates of the modes other than " + std::to_string(j) +
+                                                  " need more than 64 key bits"};
+        }
+        J = J_;
+        sampler = sampler_new_aliased(N, I, J, R, U, precision);
+        samples = mem.alloc<uint64_t>(N * J);
+        FTL_HIP_CHECK(hipMemsetAsync(samples, 0, N * J * sizeof(uint64_t), stream));
+        h = mem.alloc<double>(J * R);
+        w = mem.alloc<double>(J);
+        keys_a = mem.alloc<uint64_t>(J);
+        keys_b = mem.alloc<uint64_t>(J);
+        vals_a = mem.alloc<uint32_t>(J);
+        vals_b = mem.alloc<uint32_t>(J);
+        flags = mem.alloc<uint32_t>(J);
+        uid = mem.alloc<uint32_t>(J);
+        utup = mem.alloc<uint32_t>(J * N);
+        ufirst = mem.alloc<uint32_t>(J);
+        ustart = mem.alloc<uint32_t>(J + 1);
+        hs = mem.alloc<double>(J * R);
+        hdd = mem.alloc<double>(J * R);
+        gpart = mem.alloc<double>((uint64_t)kGramBlocks * (R * (R + 1) / 2));
+        pinv = mem.alloc<double>(R * R);
+        work = mem.alloc<double>(3 * (R + 1) * R);
+        lam = mem.alloc<double>(R);
+        mt.ensure_n(mem, J);
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void record(int i) {
+        if (!timing) return;
+        if (!ev[i]) FTL_HIP_CHECK(hipEventCreate(&ev[i]));
+        FTL_HIP_CHECK(hipEventRecord(ev[i], stream));
+    }
+
+    template <int RMAX>
+    void launch_gram(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts) {
+        hipLaunchKernelGGL((k_gram_partial<256, RMAX>), dim3(kGramBlocks), dim3(256), 0, stream, Xrows, n_dev, n_host,
+                           (uint32_t)R, parts);
+        FTL_LAUNCH_CHECK();
+    }
+
+    void gram_partials(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts) {
+        if (R <= 8) launch_gram<8>(Xrows, n_dev, n_host, parts);
+        else if (R <= 16) launch_gram<16>(Xrows, n_dev, n_host, parts);
+        else if (R <= 32) launch_gram<32>(Xrows, n_dev, n_host, parts);
+        else if (R <= 64) launch_gram<64>(Xrows, n_dev, n_host, parts);
+        else launch_gram<128>(Xrows, n_dev, n_host, parts);
+    }
+
+    // execute_ds_als_update(j, renormalize, update_sampler) (als.hpp:57-220)
+    void ds_update(uint32_t j, bool renormalize, bool update_sampler, const double* draws_dev) {
+        FTL_REQUIRE(sampler, "initialize_ds_als has not been called");
+        FTL_REQUIRE(j < N, "mode index out of range");
+        have_events = timing;
+        hipStream_t st = stream;
+        record(0);
+        // 1. draw J samples of the KRP of the modes != j
+        sampler_draw(sampler, j, samples, h, w, draws_dev, st);
+        record(1);
+        // 2. sort / unique with counts, scaled rows, sampled Gram pinv
+        const KeyLayout& DL = dlayout[j];
+        const unsigned gJ = grid_for(J, 256);
+        hipLaunchKernelGGL(k_sample_keys, dim3(gJ), dim3(256), 0, st, samples, J, DL, keys_a, vals_a);
+        FTL_LAUNCH_CHECK();
+        sort_pairs(cub, keys_a, keys_b, vals_a, vals_b, J, DL.total_bits, st);
+        hipLaunchKernelGGL(k_head_flags<uint64_t>, dim3(gJ), dim3(256), 0, st, keys_b, J, flags);
+        FTL_LAUNCH_CHECK();
+        inclusive_sum(cub, flags, uid, J, st);
+        hipLaunchKernelGGL(k_dedup_heads, dim3(gJ), dim3(256), 0, st, keys_b, vals_b, uid, J, DL, utup, ufirst,
+                           ustart, mt.counters);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_dedup_rows, dim3(grid_for(J * R, 256)), dim3(256), 0, st, ufirst, ustart, mt.counters,
+                           J, (uint32_t)R, h, w, hs, hdd);
+        FTL_LAUNCH_CHECK();
+        gram_partials(hs, mt.counters + kCntUnique, 0, gpart);
+        const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
+        hipLaunchKernelGGL(k_pinv_partials<kJacobiThreads>, dim3(1), dim3(kJacobiThreads), jac * sizeof(double), st,
+                           gpart, kGramBlocks, mt.counters, (uint32_t)R, pinv, work, lam);
+        FTL_LAUNCH_CHECK();
+        // 3. probe the unique tuples on mode j's lookup; the match count
+        //    sizes the rest, so it is read back here (one wait per update)
+        probe_and_count(*X, j, utup, mt.counters + kCntUnique, 0, J, mt, mem, cub, st);
+        FTL_HIP_CHECK(hipMemcpyAsync(mt.host_counters, mt.counters, kCntSlots * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, st));
+        FTL_HIP_CHECK(hipStreamSynchronize(st));
+        const uint64_t Jp = mt.host_counters[kCntUnique], T = mt.host_counters[kCntMatches];
+        last_unique = Jp;
+        last_found = T;
+        record(2);
+        // 4. U_j = MTTKRP(h'') pinv on the touched rows, zero elsewhere
+        FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), st));
+        if (T > 0) reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, pinv, U[j], mt, mem, cub, st);
+        else FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
+        record(3);
+        // 5. renormalize (only touched rows are nonzero) and rebuild tree j
+        if (renormalize) {
+            if (T > 0) {
+                hipLaunchKernelGGL(k_colsq_partial<256>, dim3(kColBlocks), dim3(256), 0, st, U[j], mt.seg_row,
+                                   mt.counters + kCntRows, (uint64_t)0, (uint32_t)R, colpart);
+                FTL_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(128), 0, st, colpart, kColBlocks, (uint32_t)R, norms,
+                                   sigma, 0);
+                FTL_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(T * R, 256)), dim3(256), 0, st, U[j], mt.seg_row,
+                                   mt.counters + kCntRows, (uint64_t)0, (uint32_t)R, norms);
+                FTL_LAUNCH_CHECK();
+            } else {
+                hipLaunchKernelGGL(k_fill, dim3(1), dim3(128), 0, st, sigma, R, 1.0);
+                FTL_LAUNCH_CHECK();
+            }
+        }
+        if (update_sampler) sampler_rebuild(sampler, j, st);
+        record(4);
+    }
+
+    // renormalize_columns(j); j = -1: every mode, sigma = product of norms
+    void renormalize(int j) {
+        FTL_REQUIRE(j >= -1 && j < (int)N, "mode index out of range");
+        hipStream_t st = stream;
+        if (j == -1) {
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(128), 0, st, sigma, R, 1.0);
+            FTL_LAUNCH_CHECK();
+        }
+        for (uint32_t k = 0; k < N; k++) {
+            if (j != -1 && (uint32_t)j != k) continue;
+            hipLaunchKernelGGL(k_colsq_partial<256>, dim3(kColBlocks), dim3(256), 0, st, U[k], nullptr, nullptr, I[k],
+                               (uint32_t)R, colpart);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(128), 0, st, colpart, kColBlocks, (uint32_t)R, norms,
+                               sigma, j == -1 ? 1 : 0);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(I[k] * R, 256)), dim3(256), 0, st, U[k], nullptr, nullptr,
+                               I[k], (uint32_t)R, norms);
+            FTL_LAUNCH_CHECK();
+        }
+        FTL_HIP_CHECK(hipStreamSynchronize(st));
+    }
+
+    // ||[[sigma; U]]||^2 into scal[1] (enqueued)
+    void lowrank_normsq_async() {
+        const uint64_t T = R * (R + 1) / 2;
+        for (uint32_t k = 0; k < N; k++) gram_partials(U[k], nullptr, I[k], gram_parts + (uint64_t)k * kGramBlocks * T);
+        hipLaunchKernelGGL(k_chain_final, dim3(1), dim3(256), 0, stream, gram_parts, kGramBlocks, N, (uint32_t)R, sigma,
+                           scal + 1);
+        FTL_LAUNCH_CHECK();
+    }
+
+    double lowrank_normsq() {
+        lowrank_normsq_async();
+        double out = 0;
+        FTL_HIP_CHECK(hipMemcpyAsync(&out, scal + 1, sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        return out;
+    }
+
+    template <int G, int CPL>
+    void launch_residual(const FactorPtrs& F) {
+        hipLaunchKernelGGL((k_residual_partial<G, CPL, 256>), dim3(kResidBlocks), dim3(256), 0, stream, X->keys[0],
+                           X->row[0], X->val[0], X->nnz, X->layout[0], F, sigma, (uint32_t)R, 16u, resid_part);
+        FTL_LAUNCH_CHECK();
+    }
+
+    // compute_residual_normsq(sigma, U) of the sparse tensor
+    // (sort_lookup.hpp:134-197): sum over nonzeros + ||M||^2
+    double residual_normsq() {
+        FTL_REQUIRE(X, "no sparse tensor bound to this handle");
+        FactorPtrs F{};
+        for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
+        if (R <= 32) launch_residual<32, 1>(F);
+        else if (R <= 64) launch_residual<64, 1>(F);
+        else launch_residual<64, 2>(F);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, stream, resid_part, kResidBlocks, scal);
+        FTL_LAUNCH_CHECK();
+        lowrank_normsq_async();
+        double out[2] = {0, 0};
+        FTL_HIP_CHECK(hipMemcpyAsync(out, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        return out[0] + out[1];
+    }
+
+    void stats(double* out) {
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        FTL_HIP_CHECK(hipMemcpy(mt.host_counters, mt.counters, kCntSlots * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        out[0] = (double)last_unique;
+        out[1] = (double)last_found;
+        out[2] = last_found ? (double)mt.host_counters[kCntRows] : 0.0;
+        out[3] = last_found ? (double)mt.host_counters[kCntPieces] : 0.0;
+        for (int i = 0; i < 4; i++) {
+            float t = 0.f;
+            if (have_events && ev[i] && ev[i + 1]) FTL_HIP_CHECK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            out[4 + i] = have_events ? (double)t : 0.0;
+        }
+    }
+};
+
+}  // namespace ftl
+
+struct ftl_sparse {
+    ftl::SparseDev impl;
+};
+
+struct ftl_als {
+    ftl::Als impl;
+};
+
+using ftl::guarded;
+
+extern "C" {
+
+static int sparse_create_common(uint32_t N, uint64_t nnz, const uint32_t* idx, const double* values,
+                                const uint64_t* dims, bool on_device, ftl_sparse** out) {
+    return guarded([&] {
+        FTL_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<ftl_sparse> s(new ftl_sparse());
+        s->impl.build(N, nnz, idx, values, dims, on_device);
+        *out = s.release();
+    });
+}
+
+int ftl_sparse_create(uint32_t N, uint64_t nnz, const uint32_t* idx_host, const double* values_host,
+                      const uint64_t* dims, ftl_sparse** out) {
+    return sparse_create_common(N, nnz, idx_host, values_host, dims, false, out);
+}
+
+int ftl_sparse_create_device(uint32_t N, uint64_t nnz, const uint32_t* idx_dev, const double* values_dev,
+                             const uint64_t* dims, ftl_sparse** out) {
+    return sparse_create_common(N, nnz, idx_dev, values_dev, dims, true, out);
+}
+
+int ftl_sparse_destroy(ftl_sparse* X) {
+    return guarded([&] { delete X; });
+}
+
+int ftl_sparse_info(ftl_sparse* X, uint64_t* out) {
+    return guarded([&] {
+        FTL_REQUIRE(X && out, "null argument");
+        out[0] = X->impl.N;
+        out[1] = X->impl.nnz;
+        for (uint32_t k = 0; k < X->impl.N; k++) out[2 + k] = X->impl.dims[k];
+    });
+}
+
+int ftl_sparse_normsq(ftl_sparse* X, double* out) {
+    return guarded([&] {
+        FTL_REQUIRE(X && out, "null argument");
+        *out = X->impl.normsq;
+    });
+}
+
+// Probes host samples (Jp x N uint64) on mode j: leaves ranges and the match
+// total in the tensor's match buffers; returns T.
+static uint64_t probe_host_samples(ftl::SparseDev& S, uint32_t j, uint64_t Jp, const uint64_t* samples_host) {
+    using namespace ftl;
+    FTL_REQUIRE(j < S.N, "mode index out of range");
+    FTL_REQUIRE(Jp >= 1 && Jp < (1ull << 31), "sample count must be in [1, 2^31)");
+    FTL_REQUIRE(samples_host, "null samples");
+    hipStream_t st = S.stream;
+    uint32_t* tup = S.tuples(Jp);
+    uint64_t* stage = S.mem.alloc<uint64_t>(Jp * S.N);
+    FTL_HIP_CHECK(hipMemcpyAsync(stage, samples_host, Jp * S.N * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_tuples_u32, dim3(grid_for(Jp * S.N, 256)), dim3(256), 0, st, stage, Jp * S.N, tup);
+    FTL_HIP_CHECK(hipGetLastError());
+    probe_and_count(S, j, tup, nullptr, Jp, Jp, S.mt, S.mem, S.cub, st);
+    FTL_HIP_CHECK(hipMemcpyAsync(S.mt.host_counters, S.mt.counters, kCntSlots * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, st));
+    FTL_HIP_CHECK(hipStreamSynchronize(st));
+    S.mem.release(stage);
+    return S.mt.host_counters[kCntMatches];
+}
+
+int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* samples_host, const double* lhs_host,
+                      uint64_t R, double* result_host, uint64_t result_rows, uint64_t* found) {
+    return guarded([&] {
+        using namespace ftl;
+        FTL_REQUIRE(X && lhs_host && result_host, "null argument");
+        SparseDev& S = X->impl;
+        FTL_REQUIRE(R >= 1 && R <= 128, "R must be in [1, 128]");
+        FTL_REQUIRE(j < S.N, "mode index out of range");
+        FTL_REQUIRE(result_rows >= S.dims[j], "result has fewer rows than the tensor's mode size");
+        const uint64_t T = probe_host_samples(S, j, Jp, samples_host);
+        hipStream_t st = S.stream;
+        Arena tmp;
+        StreamDrain drain{st};
+        double* lhs = tmp.alloc<double>(Jp * R);
+        double* res = tmp.alloc<double>(result_rows * R);
+        FTL_HIP_CHECK(hipMemcpyAsync(lhs, lhs_host, Jp * R * sizeof(double), hipMemcpyHostToDevice, st));
+        FTL_HIP_CHECK(hipMemsetAsync(res, 0, result_rows * R * sizeof(double), st));
+        if (T > 0) reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, nullptr, res, S.mt, S.mem, S.cub, st);
+        FTL_HIP_CHECK(hipMemcpyAsync(result_host, res, result_rows * R * sizeof(double), hipMemcpyDeviceToHost, st));
+        FTL_HIP_CHECK(hipStreamSynchronize(st));
+        if (found) *found = T;
+    });
+}
+
+int ftl_sparse_lookup(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* samples_host, uint64_t* counts_host,
+                      uint64_t* ids_host, uint64_t ids_cap, uint64_t* found) {
+    return guarded([&] {
+        using namespace ftl;
+        FTL_REQUIRE(X, "null sparse tensor");
+        SparseDev& S = X->impl;
+        const uint64_t T = probe_host_samples(S, j, Jp, samples_host);
+        hipStream_t st = S.stream;
+        if (counts_host) {
+            std::vector<uint32_t> c(Jp);
+            FTL_HIP_CHECK(hipMemcpyAsync(c.data(), S.mt.cnt, Jp * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            FTL_HIP_CHECK(hipStreamSynchronize(st));
+            for (uint64_t s = 0; s < Jp; s++) counts_host[s] = c[s];
+        }
+        if (ids_host && T > 0) {
+            Arena tmp;
+            StreamDrain drain{st};
+            uint64_t* ids = tmp.alloc<uint64_t>(T);
+            hipLaunchKernelGGL(k_match_ids, dim3(grid_for(T, 256)), dim3(256), 0, st, S.mt.off, S.mt.lo, Jp, T,
+                               S.perm[j], ids);
+            FTL_HIP_CHECK(hipGetLastError());
+            FTL_HIP_CHECK(hipMemcpyAsync(ids_host, ids, std::min(T, ids_cap) * sizeof(uint64_t),
+                                         hipMemcpyDeviceToHost, st));
+            FTL_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        if (found) *found = T;
+    });
+}
+
+static int als_create_common(ftl_sparse* X, uint32_t N, const uint64_t* I, uint64_t R, const double* const* U_host,
+                             ftl_als** out) {
+    return guarded([&] {
+        FTL_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<ftl_als> a(new ftl_als());
+        a->impl.init(X ? &X->impl : nullptr, N, I, R, U_host);
+        *out = a.release();
+    });
+}
+
+int ftl_sparse_residual_normsq(ftl_sparse* X, uint64_t R, const uint64_t* I, const double* sigma_host,
+                               const double* const* U_host, double* out) {
+    return guarded([&] {
+        FTL_REQUIRE(X && sigma_host && out, "null argument");
+        ftl::Als a;
+        a.init(&X->impl, X->impl.N, I, R, U_host);
+        FTL_HIP_CHECK(hipMemcpy(a.sigma, sigma_host, R * sizeof(double), hipMemcpyHostToDevice));
+        *out = a.residual_normsq();
+    });
+}
+
+int ftl_als_create(ftl_sparse* X, uint32_t N, const uint64_t* I, uint64_t R, const double* const* U_host,
+                   ftl_als** out) {
+    return als_create_common(X, N, I, R, U_host, out);
+}
+
+int ftl_als_destroy(ftl_als* a) {
+    return guarded([&] { delete a; });
+}
+
+int ftl_als_initialize_ds(ftl_als* a, uint64_t J, int precision) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        FTL_REQUIRE(precision == FTL_FP64 || precision == FTL_FP32, "precision must be FTL_FP64 or FTL_FP32");
+        a->impl.initialize_ds(J, precision);
+    });
+}
+
+int ftl_als_set_seed(ftl_als* a, uint64_t seed, uint64_t counter) {
+    return guarded([&] {
+        FTL_REQUIRE(a && a->impl.sampler, "initialize_ds_als has not been called");
+        ftl::sampler_set_seed(a->impl.sampler, seed, counter);
+    });
+}
+
+int ftl_als_ds_update_device(ftl_als* a, uint32_t j, int renormalize, int update_sampler, const double* draws_dev) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        a->impl.ds_update(j, renormalize != 0, update_sampler != 0, draws_dev);
+    });
+}
+
+int ftl_als_ds_update(ftl_als* a, uint32_t j, int renormalize, int update_sampler, const double* draws_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        ftl::Als& A = a->impl;
+        FTL_REQUIRE(A.sampler, "initialize_ds_als has not been called");
+        const double* d = nullptr;
+        if (draws_host) {
+            if (!A.draws_stage) A.draws_stage = A.mem.alloc<double>(2 * A.N * A.J);
+            FTL_HIP_CHECK(hipMemcpyAsync(A.draws_stage, draws_host, 2 * A.N * A.J * sizeof(double),
+                                         hipMemcpyHostToDevice, A.stream));
+            d = A.draws_stage;
+        }
+        A.ds_update(j, renormalize != 0, update_sampler != 0, d);
+        FTL_HIP_CHECK(hipStreamSynchronize(A.stream));
+    });
+}
+
+int ftl_als_get_factor(ftl_als* a, uint32_t k, double* U_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a && U_host, "null argument");
+        ftl::Als& A = a->impl;
+        FTL_REQUIRE(k < A.N, "mode index out of range");
+        FTL_HIP_CHECK(hipMemcpyAsync(U_host, A.U[k], A.I[k] * A.R * sizeof(double), hipMemcpyDeviceToHost, A.stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(A.stream));
+    });
+}
+
+int ftl_als_set_factor(ftl_als* a, uint32_t k, const double* U_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a && U_host, "null argument");
+        ftl::Als& A = a->impl;
+        FTL_REQUIRE(k < A.N, "mode index out of range");
+        FTL_HIP_CHECK(hipMemcpyAsync(A.U[k], U_host, A.I[k] * A.R * sizeof(double), hipMemcpyHostToDevice, A.stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(A.stream));
+    });
+}
+
+int ftl_als_update_sampler(ftl_als* a, uint32_t k) {
+    return guarded([&] {
+        FTL_REQUIRE(a && a->impl.sampler, "initialize_ds_als has not been called");
+        ftl::sampler_rebuild(a->impl.sampler, k, a->impl.stream);
+        FTL_HIP_CHECK(hipStreamSynchronize(a->impl.stream));
+    });
+}
+
+int ftl_als_get_sigma(ftl_als* a, double* sigma_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a && sigma_host, "null argument");
+        FTL_HIP_CHECK(hipMemcpyAsync(sigma_host, a->impl.sigma, a->impl.R * sizeof(double), hipMemcpyDeviceToHost,
+                                     a->impl.stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(a->impl.stream));
+    });
+}
+
+int ftl_als_set_sigma(ftl_als* a, const double* sigma_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a && sigma_host, "null argument");
+        FTL_HIP_CHECK(hipMemcpyAsync(a->impl.sigma, sigma_host, a->impl.R * sizeof(double), hipMemcpyHostToDevice,
+                                     a->impl.stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(a->impl.stream));
+    });
+}
+
+int ftl_als_renormalize_columns(ftl_als* a, int j) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        a->impl.renormalize(j);
+    });
+}
+
+int ftl_als_residual_normsq(ftl_als* a, double* out) {
+    return guarded([&] {
+        FTL_REQUIRE(a && out, "null argument");
+        *out = a->impl.residual_normsq();
+    });
+}
+
+int ftl_als_lowrank_normsq(ftl_als* a, double* out) {
+    return guarded([&] {
+        FTL_REQUIRE(a && out, "null argument");
+        *out = a->impl.lowrank_normsq();
+    });
+}
+
+int ftl_als_factor_device(ftl_als* a, uint32_t k, double** out) {
+    return guarded([&] {
+        FTL_REQUIRE(a && out, "null argument");
+        FTL_REQUIRE(k < a->impl.N, "mode index out of range");
+        *out = a->impl.U[k];
+    });
+}
+
+int ftl_als_last_stats(ftl_als* a, double* out8) {
+    return guarded([&] {
+        FTL_REQUIRE(a && out8, "null argument");
+        a->impl.stats(out8);
+    });
+}
+
+int ftl_als_enable_timing(ftl_als* a, int enable) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        a->impl.timing = enable != 0;
+    });
+}
+
+void* ftl_als_stream(ftl_als* a) { return a ? (void*)a->impl.stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,675 @@
+// als_kernels.hpp -- gfx950 kernels of the sparse tensor and the sampled
+// CP-ALS step.
+//
+//   k_nz_keys / k_nz_gather    per-mode sorted lookup of the nonzeros
+//                              (SortIdxLookup ctor, sort_lookup.hpp:33-60)
+//   k_sample_keys ...          sort/unique of the drawn samples with counts
+//   k_dedup_rows               w' = w c, h' = sqrt(w') h, h'' = sqrt(w') h'
+//                              (als.hpp:76-160)
+//   k_gram_partial             sampled Gram h'^T h' and factor Grams U^T U
+//   k_pinv_partials            eigendecomposition-based pinv with the 1e-11
+//                              cut and target rank (common.h:228-323)
+//   k_probe / k_expand         equal_range of every sample on the sorted
+//                              lookup and the list of matched nonzeros
+//                              (sort_lookup.hpp:69-132)
+//   k_seg_* / k_item_partial   sampled MTTKRP as a segmented reduction over
+//   k_seg_final                matches sorted by output row, fused with the
+//                              product by pinv (als.hpp:189-203)
+//   k_colsq_partial ...        renormalize_columns (low_rank_tensor.hpp:203-260)
+//   k_residual_partial         sum over nonzeros of x^2 - 2 x m
+//   k_chain_final              ||M||^2 = sum sigma sigma^T (*) prod_k U_k^T U_k
+//                              (sort_lookup.hpp:134-197, common.h:152-226)
+//
+// Every reduction has a fixed association (block partials summed in block
+// order), so results do not depend on scheduling; there are no floating-point
+// atomics.  A sum over the matches of one output row is taken sequentially
+// in (sample, nonzero) order when the row has at most kSegPiece matches --
+// the order of the CPU restatement, so those rows agree bitwise -- and in
+// pieces of kSegPiece otherwise.
+#pragma once
+
+#include "ftl_common.hpp"
+#include "jacobi.hpp"
+#include "wave_ops.hpp"
+
+namespace ftl {
+
+// Matches of one output row summed sequentially per piece.
+constexpr uint32_t kSegPiece = 256;
+// Blocks of the partial-sum kernels (fixed, so the association is fixed).
+constexpr uint32_t kGramBlocks = 128;
+constexpr uint32_t kColBlocks = 256;
+constexpr uint32_t kResidBlocks = 1024;
+
+// Coordinates of every mode but `skip` packed into one 64-bit key, the
+// lowest-numbered mode in the most significant field, so that key order is
+// the lexicographic order over modes != skip of the reference comparators
+// (sort_lookup.hpp:52-58, als.hpp:92-101).
+struct KeyLayout {
+    uint32_t N;
+    uint32_t skip;
+    uint32_t shift[kMaxModes];
+    uint32_t bits[kMaxModes];
+    uint32_t total_bits;
+};
+
+struct FactorPtrs {
+    const double* U[kMaxModes];
+};
+
+// Counters the ALS pipeline keeps on the device: unique samples J', matched
+// nonzeros T, touched rows D, reduction pieces W.
+enum { kCntUnique = 0, kCntMatches = 1, kCntRows = 2, kCntPieces = 3, kCntSlots = 4 };
+
+__device__ inline uint64_t pack_tuple(const KeyLayout& L, const uint32_t* t) {
+    uint64_t key = 0;
+    for (uint32_t k = 0; k < L.N; k++)
+        if (k != L.skip) key |= (uint64_t)t[k] << L.shift[k];
+    return key;
+}
+
+__device__ inline uint32_t key_field(const KeyLayout& L, uint64_t key, uint32_t k) {
+    const uint32_t b = L.bits[k];
+    return b == 0 ? 0u : (uint32_t)((key >> L.shift[k]) & (b >= 64 ? ~0ull : ((1ull << b) - 1)));
+}
+
+// ---------------------------------------------------------------------------
+// sparse tensor construction
+// ---------------------------------------------------------------------------
+__global__ void k_coord_max(const uint32_t* __restrict__ idx, uint64_t nnz, uint32_t N, uint32_t* __restrict__ mx) {
+    __shared__ uint32_t loc[kMaxModes];
+    if (threadIdx.x < kMaxModes) loc[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t m[kMaxModes];
+    for (uint32_t k = 0; k < N; k++) m[k] = 0;
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x)
+        for (uint32_t k = 0; k < N; k++) m[k] = max(m[k], idx[z * N + k]);
+    for (uint32_t k = 0; k < N; k++) atomicMax(&loc[k], m[k]);
+    __syncthreads();
+    if (threadIdx.x < N) atomicMax(&mx[threadIdx.x], loc[threadIdx.x]);
+}
+
+__global__ void k_nz_keys(const uint32_t* __restrict__ idx, uint64_t nnz, KeyLayout L, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ perm) {
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz;
+         z += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t key = 0;
+        for (uint32_t k = 0; k < L.N; k++)
+            if (k != L.skip) key |= (uint64_t)idx[z * L.N + k] << L.shift[k];
+        keys[z] = key;
+        perm[z] = (uint32_t)z;
+    }
+}
+
+__global__ void k_nz_gather(const uint32_t* __restrict__ idx, const double* __restrict__ val,
+                            const uint32_t* __restrict__ perm, uint64_t nnz, uint32_t N, uint32_t j,
+                            uint32_t* __restrict__ row, double* __restrict__ vout) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t z = perm[i];
+        row[i] = idx[z * N + j];
+        vout[i] = val[z];
+    }
+}
+
+// Block partial sums of v^2 (fixed grid; summed in block order by k_sum_final).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_sumsq_partial(const double* __restrict__ v, uint64_t n,
+                                                      double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double red[BS];
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BS)
+        acc += v[i] * v[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_final(const double* __restrict__ partial, uint32_t n, double* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < n; i++) s += partial[i];
+        out[0] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sample sort / unique (als.hpp:76-160)
+// ---------------------------------------------------------------------------
+// samples: N x J (row k = mode k, the sampler's layout); coordinates cast to
+// uint32 as the reference's transpose does (als.hpp:79).
+__global__ void k_sample_keys(const uint64_t* __restrict__ samples, uint64_t J, KeyLayout L,
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < J; s += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t key = 0;
+        for (uint32_t k = 0; k < L.N; k++)
+            if (k != L.skip) key |= (uint64_t)(uint32_t)samples[k * J + s] << L.shift[k];
+        keys[s] = key;
+        vals[s] = (uint32_t)s;
+    }
+}
+
+template <typename K>
+__global__ void k_head_flags(const K* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// At each run head of the sorted sample keys: the unique tuple (coordinates
+// of its first sample, mode `skip` zero), that first sample and the run
+// start.  The last thread closes the run list and stores J'.
+__global__ void k_dedup_heads(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
+                              const uint32_t* __restrict__ uid, uint64_t J, KeyLayout L,
+                              uint32_t* __restrict__ utup, uint32_t* __restrict__ ufirst,
+                              uint32_t* __restrict__ ustart, uint32_t* __restrict__ counters) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < J; i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool head = i == 0 || skeys[i] != skeys[i - 1];
+        if (head) {
+            const uint32_t u = uid[i] - 1;
+            ufirst[u] = svals[i];
+            ustart[u] = (uint32_t)i;
+            for (uint32_t k = 0; k < L.N; k++) utup[(uint64_t)u * L.N + k] = key_field(L, skeys[i], k);
+        }
+        if (i == J - 1) {
+            ustart[uid[i]] = (uint32_t)J;
+            counters[kCntUnique] = uid[i];
+        }
+    }
+}
+
+// w' = w[first] * count, h' = h[first] * sqrt(w'), h'' = h' * sqrt(w')
+// (als.hpp:136-170: the same two roundings).
+__global__ void k_dedup_rows(const uint32_t* __restrict__ ufirst, const uint32_t* __restrict__ ustart,
+                             const uint32_t* __restrict__ counters, uint64_t J, uint32_t R,
+                             const double* __restrict__ h, const double* __restrict__ w,
+                             double* __restrict__ hs, double* __restrict__ hdd) {
+#pragma clang fp contract(off)
+    const uint64_t Jp = counters[kCntUnique];
+    const uint64_t n = Jp * R;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t u = e / R, c = e - u * R;
+        const uint64_t f = ufirst[u];
+        const double wd = w[f] * (double)(ustart[u + 1] - ustart[u]);
+        const double sq = sqrt(wd);
+        const double a = h[f * R + c] * sq;
+        hs[e] = a;
+        hdd[e] = a * sq;
+    }
+    (void)J;
+}
+
+// ---------------------------------------------------------------------------
+// Gram of the rows of X (n x R, row-major): block b sums rows
+// [b*chunk, (b+1)*chunk) sequentially into the packed upper triangle
+// partial[b][0..T).  n comes from *n_dev when given.
+// ---------------------------------------------------------------------------
+template <int BS, int RMAX>
+__global__ void __launch_bounds__(BS) k_gram_partial(const double* __restrict__ X, const uint32_t* n_dev,
+                                                     uint64_t n_host, uint32_t R, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    constexpr int TMAX = RMAX * (RMAX + 1) / 2;
+    constexpr int EPT = (TMAX + BS - 1) / BS;
+    constexpr int TILE = 32;
+    __shared__ double tile[TILE * RMAX];
+    __shared__ uint8_t pa[TMAX], pb[TMAX];
+    const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
+    const int T = (int)(R * (R + 1) / 2);
+    for (int r = threadIdx.x; r < (int)R; r += BS) {
+        const int base = r * (int)R - r * (r - 1) / 2;
+        for (int c = r; c < (int)R; c++) {
+            pa[base + c - r] = (uint8_t)r;
+            pb[base + c - r] = (uint8_t)c;
+        }
+    }
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = min<uint64_t>(n, (uint64_t)blockIdx.x * chunk), r1 = min<uint64_t>(n, r0 + chunk);
+    double acc[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) acc[q] = 0.0;
+    __syncthreads();
+    for (uint64_t t0 = r0; t0 < r1; t0 += TILE) {
+        const int rows = (int)min<uint64_t>(TILE, r1 - t0);
+        for (int e = threadIdx.x; e < rows * (int)R; e += BS) tile[e] = X[t0 * R + e];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            const int e = threadIdx.x + q * BS;
+            if (e < T) {
+                const int a = pa[e], b = pb[e];
+                double s = acc[q];
+                for (int r = 0; r < rows; r++) s += tile[r * R + a] * tile[r * R + b];
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int e = threadIdx.x + q * BS;
+        if (e < T) partial[(uint64_t)blockIdx.x * T + e] = acc[q];
+    }
+}
+
+// compute_pinv over the block partials of the sampled Gram: M = sum of the
+// partials (in block order), eigendecomposition, keep eigenvalue v when
+// v >= R - target and lambda_v > 1e-11 with target = min(J', R), then
+// pinv = M' M'^T with M'[:, v] = V[:, v] / sqrt(lambda_v) (common.h:228-323).
+// One block; pinv written full symmetric (R x R).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_pinv_partials(const double* __restrict__ partial, uint32_t nparts,
+                                                      const uint32_t* __restrict__ counters, uint32_t R,
+                                                      double* __restrict__ pinv, double* __restrict__ work,
+                                                      double* __restrict__ lam_out) {
+#pragma clang fp contract(off)
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double lam[128], diag_tmp[128];
+    const int tid = threadIdx.x;
+    const int Ri = (int)R, LD = Ri + 1, T = Ri * (Ri + 1) / 2;
+    JacobiStore st = jacobi_store(smem, work, Ri);
+    for (int e = tid; e < T; e += BS) {
+        // packed index e -> (a, b), a <= b
+        int a = 0, base = 0;
+        while (base + (Ri - a) <= e) {
+            base += Ri - a;
+            a++;
+        }
+        const int b = a + (e - base);
+        double s = 0.0;
+        for (uint32_t p = 0; p < nparts; p++) s += partial[(uint64_t)p * T + e];
+        st.A[a * LD + b] = s;
+        st.A[b * LD + a] = s;
+    }
+    __syncthreads();
+    jacobi_block<BS>(st.A, st.V, Ri, lam, st.vecs, js, diag_tmp);
+    const uint32_t Jp = counters[kCntUnique];
+    const int target = (int)min<uint32_t>(Jp, R);
+    // M' in place of the eigenvectors
+    for (int w = tid; w < Ri * Ri; w += BS) {
+        const int u = w / Ri, v = w - u * Ri;
+        const bool keep = v >= Ri - target && lam[v] > 1e-11;
+        st.vecs[u * Ri + v] = keep ? st.vecs[u * Ri + v] / sqrt(lam[v]) : 0.0;
+    }
+    if (tid < Ri) lam_out[tid] = lam[tid];
+    __syncthreads();
+    for (int w = tid; w < Ri * Ri; w += BS) {
+        const int a = w / Ri, b = w - a * Ri;
+        if (a > b) continue;
+        double s = 0.0;
+        for (int c = 0; c < Ri; c++) s += st.vecs[a * Ri + c] * st.vecs[b * Ri + c];
+        pinv[a * Ri + b] = s;
+        pinv[b * Ri + a] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// probe: equal_range of each tuple's key on mode j's sorted keys
+// (sort_lookup.hpp:94-103).  A coordinate outside the tensor's range cannot
+// match (its field would not hold it) and gives an empty range.
+// ---------------------------------------------------------------------------
+struct ProbeArgs {
+    const uint32_t* tup;       // n x N, row-major
+    const uint32_t* n_dev;     // J' on the device, or null (n_host)
+    uint64_t n_host;
+    uint64_t n_cap;            // entries of lo / cnt written (zero past n)
+    const uint64_t* keys;      // sorted lookup keys of mode j
+    uint64_t nnz;
+    KeyLayout L;               // the tensor's layout for mode j
+    uint64_t dims[kMaxModes];  // coordinate bound per mode
+    uint32_t* lo;
+    uint32_t* cnt;
+};
+
+__global__ void k_probe(ProbeArgs a) {
+    const uint64_t n = a.n_dev ? (uint64_t)*a.n_dev : a.n_host;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n_cap;
+         s += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, c = 0;
+        if (s < n) {
+            const uint32_t* t = a.tup + s * a.L.N;
+            bool ok = true;
+            uint64_t key = 0;
+            for (uint32_t k = 0; k < a.L.N; k++) {
+                if (k == a.L.skip) continue;
+                ok &= t[k] < a.dims[k];
+                key |= (uint64_t)t[k] << a.L.shift[k];
+            }
+            if (ok) {
+                uint64_t l = 0, h = a.nnz;
+                while (l < h) {  // first position with keys[p] >= key
+                    const uint64_t m = (l + h) >> 1;
+                    if (a.keys[m] < key) l = m + 1;
+                    else h = m;
+                }
+                uint64_t l2 = l, h2 = a.nnz;
+                while (l2 < h2) {  // first position with keys[p] > key
+                    const uint64_t m = (l2 + h2) >> 1;
+                    if (a.keys[m] <= key) l2 = m + 1;
+                    else h2 = m;
+                }
+                lo = (uint32_t)l;
+                c = (uint32_t)(l2 - l);
+            }
+        }
+        a.lo[s] = lo;
+        a.cnt[s] = c;
+    }
+}
+
+// counters[slot] = off[n-1] + cnt[n-1] (n >= 1): total of an exclusive scan.
+__global__ void k_scan_total(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint64_t n,
+                             uint32_t* __restrict__ counters, int slot) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[slot] = off[n - 1] + cnt[n - 1];
+}
+
+// Match m (sample u's (m - off[u])-th nonzero of its range): output row and
+// payload (u << 32 | sorted position).  Matches are numbered in (sample,
+// position) order.
+__global__ void k_expand(const uint32_t* __restrict__ off, const uint32_t* __restrict__ lo, uint64_t n,
+                         uint64_t T, const uint32_t* __restrict__ rowj, uint32_t* __restrict__ mrow,
+                         uint64_t* __restrict__ mpay) {
+    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < T; m += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t l = 0, h = n;  // largest u with off[u] <= m
+        while (h - l > 1) {
+            const uint64_t mid = (l + h) >> 1;
+            if (off[mid] <= m) l = mid;
+            else h = mid;
+        }
+        const uint64_t pos = (uint64_t)lo[l] + (m - off[l]);
+        mrow[m] = rowj[pos];
+        mpay[m] = (l << 32) | pos;
+    }
+}
+
+// Row segments of the matches sorted by row: start of segment d, its row,
+// and the number of reduction pieces (ceil(len / kSegPiece)).
+__global__ void k_seg_heads(const uint32_t* __restrict__ srow, const uint32_t* __restrict__ segid, uint64_t T,
+                            uint32_t* __restrict__ seg_start, uint32_t* __restrict__ seg_row,
+                            uint32_t* __restrict__ counters) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i == 0 || srow[i] != srow[i - 1]) {
+            const uint32_t d = segid[i] - 1;
+            seg_start[d] = (uint32_t)i;
+            seg_row[d] = srow[i];
+        }
+        if (i == T - 1) {
+            seg_start[segid[i]] = (uint32_t)T;
+            counters[kCntRows] = segid[i];
+        }
+    }
+}
+
+__global__ void k_seg_pieces(const uint32_t* __restrict__ seg_start, const uint32_t* __restrict__ counters,
+                             uint64_t cap, uint32_t* __restrict__ npieces) {
+    const uint64_t D = counters[kCntRows];
+    for (uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; d < cap; d += (uint64_t)gridDim.x * blockDim.x)
+        npieces[d] = d < D ? (seg_start[d + 1] - seg_start[d] + kSegPiece - 1) / kSegPiece : 0u;
+}
+
+// Piece w of the segmented reduction: the sequential sum over its matches of
+// lhs[u] * value (a group of G lanes, CPL columns per lane).
+template <int G, int CPL>
+__global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict__ seg_start,
+                                                      const uint32_t* __restrict__ piece_off,
+                                                      const uint32_t* __restrict__ counters,
+                                                      const uint64_t* __restrict__ spay,
+                                                      const double* __restrict__ valj,
+                                                      const double* __restrict__ lhs, uint32_t R,
+                                                      double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t lane = threadIdx.x % G;
+    const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
+    for (uint64_t w = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; w < W; w += groups) {
+        uint32_t l = 0, h = D;  // segment of piece w: largest d with piece_off[d] <= w
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (piece_off[mid] <= w) l = mid;
+            else h = mid;
+        }
+        const uint64_t i0 = (uint64_t)seg_start[l] + (uint64_t)(w - piece_off[l]) * kSegPiece;
+        const uint64_t i1 = min<uint64_t>(i0 + kSegPiece, seg_start[l + 1]);
+        double acc[CPL];
+#pragma unroll
+        for (int q = 0; q < CPL; q++) acc[q] = 0.0;
+        for (uint64_t i = i0; i < i1; i++) {
+            const uint64_t pay = spay[i];
+            const uint64_t u = pay >> 32;
+            const double v = valj[pay & 0xffffffffull];
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const uint32_t c = lane + q * G;
+                if (c < R) acc[q] = acc[q] + lhs[u * R + c] * v;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const uint32_t c = lane + q * G;
+            if (c < R) partial[(uint64_t)w * R + c] = acc[q];
+        }
+    }
+}
+
+// Segment d: sum of its pieces in order, then either the row itself (plain
+// MTTKRP) or its product with pinv (the ALS solve U_j = MTTKRP pinv,
+// als.hpp:189-203), written to row seg_row[d] of out.
+template <int G, int CPL, bool SOLVE>
+__global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ seg_row,
+                                                   const uint32_t* __restrict__ piece_off,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const double* __restrict__ partial, uint32_t R,
+                                                   const double* __restrict__ pinv, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    constexpr int GPB = 256 / G;
+    __shared__ double mt_s[GPB][G * CPL];
+    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
+    const uint64_t groups = (uint64_t)gridDim.x * GPB;
+    // every group runs the same number of iterations (block-uniform barrier)
+    const uint64_t iters = (D + groups - 1) / groups;
+    for (uint64_t it = 0; it < iters; it++) {
+        const uint64_t d = it * groups + (uint64_t)blockIdx.x * GPB + gi;
+        const bool valid = d < D;
+        double mt[CPL];
+        if (valid) {
+            const uint32_t p0 = piece_off[d], p1 = d + 1 < D ? piece_off[d + 1] : W;
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const uint32_t c = lane + q * G;
+                double s = 0.0;
+                if (c < R) {
+                    s = partial[(uint64_t)p0 * R + c];
+                    for (uint32_t p = p0 + 1; p < p1; p++) s = s + partial[(uint64_t)p * R + c];
+                }
+                mt[q] = s;
+            }
+        }
+        if constexpr (!SOLVE) {
+            if (valid) {
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    if (c < R) out[(uint64_t)seg_row[d] * R + c] = mt[q];
+                }
+            }
+        } else {
+            if (valid) {
+#pragma unroll
+                for (int q = 0; q < CPL; q++) mt_s[gi][lane + q * G] = mt[q];
+            }
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    if (c < R) {
+                        double s = 0.0;
+                        for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * pinv[k * R + c];
+                        out[(uint64_t)seg_row[d] * R + c] = s;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column norms (renormalize_columns, low_rank_tensor.hpp:203-260).  Rows are
+// rows[0..n) when rows is given (n from *n_dev or n_host), else 0..n.
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ void __launch_bounds__(BS) k_colsq_partial(const double* __restrict__ U, const uint32_t* __restrict__ rows,
+                                                      const uint32_t* n_dev, uint64_t n_host, uint32_t R,
+                                                      double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double red[BS];
+    const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
+    const uint32_t lanes = (BS / R);  // rows in flight per block
+    const uint32_t rl = threadIdx.x / R, c = threadIdx.x - rl * R;
+    double acc = 0.0;
+    if (rl < lanes) {
+        for (uint64_t i = (uint64_t)blockIdx.x * lanes + rl; i < n; i += (uint64_t)gridDim.x * lanes) {
+            const uint64_t r = rows ? (uint64_t)rows[i] : i;
+            const double x = U[r * R + c];
+            acc += x * x;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < R) {
+        double s = 0.0;
+        for (uint32_t q = 0; q < lanes; q++) s += red[q * R + threadIdx.x];
+        partial[(uint64_t)blockIdx.x * R + threadIdx.x] = s;
+    }
+}
+
+// norms[c] = sqrt(sum of the partials), 1 when <= 1e-7; sigma either the
+// norms (mode_only) or sigma * norms.
+__global__ void k_colnorm_final(const double* __restrict__ partial, uint32_t nparts, uint32_t R,
+                                double* __restrict__ norms, double* __restrict__ sigma, int multiply) {
+#pragma clang fp contract(off)
+    for (uint32_t c = threadIdx.x; c < R; c += blockDim.x) {
+        double s = 0.0;
+        for (uint32_t p = 0; p < nparts; p++) s += partial[(uint64_t)p * R + c];
+        double nrm = sqrt(s);
+        if (nrm <= 1e-7) nrm = 1.0;
+        norms[c] = nrm;
+        if (sigma) sigma[c] = multiply ? sigma[c] * nrm : nrm;
+    }
+}
+
+__global__ void k_scale_rows(double* __restrict__ U, const uint32_t* __restrict__ rows, const uint32_t* n_dev,
+                             uint64_t n_host, uint32_t R, const double* __restrict__ norms) {
+    const uint64_t n = (n_dev ? (uint64_t)*n_dev : n_host) * R;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / R, c = e - i * R;
+        const uint64_t r = rows ? (uint64_t)rows[i] : i;
+        U[r * R + c] = U[r * R + c] / norms[c];
+    }
+}
+
+__global__ void k_fill(double* __restrict__ p, uint64_t n, double v) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// residual: sum over nonzeros (mode-0 lookup order) of x^2 - 2 x m, with
+// m = sum_r sigma_r prod_k U_k[i_k][r]; the partial product over modes 1..N-1
+// is reused while consecutive nonzeros share those coordinates
+// (sort_lookup.hpp:134-197).  A group of G lanes walks a run of consecutive
+// nonzeros, lane r holding column r (CPL per lane).
+// ---------------------------------------------------------------------------
+template <int G, int CPL, int BS>
+__global__ void __launch_bounds__(BS) k_residual_partial(const uint64_t* __restrict__ keys0,
+                                                         const uint32_t* __restrict__ row0,
+                                                         const double* __restrict__ val0, uint64_t nnz,
+                                                         KeyLayout L, FactorPtrs F, const double* __restrict__ sigma,
+                                                         uint32_t R, uint32_t run, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    constexpr int GPB = BS / G;
+    __shared__ double red[GPB];
+    const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
+    const uint64_t groups = (uint64_t)gridDim.x * GPB;
+    double acc = 0.0;  // identical in every lane of a group
+    for (uint64_t r0 = ((uint64_t)blockIdx.x * GPB + gi) * run; r0 < nnz; r0 += groups * run) {
+        const uint64_t r1 = min<uint64_t>(nnz, r0 + run);
+        double pp[CPL];
+        uint64_t prev = 0;
+        for (uint64_t i = r0; i < r1; i++) {
+            const uint64_t key = keys0[i];
+            if (i == r0 || key != prev) {
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    double p = 0.0;
+                    if (c < R) {
+                        p = sigma[c];
+                        for (uint32_t k = 1; k < L.N; k++)
+                            p = p * F.U[k][(uint64_t)key_field(L, key, k) * R + c];
+                    }
+                    pp[q] = p;
+                }
+                prev = key;
+            }
+            const uint64_t i0 = row0[i];
+            double part = 0.0;
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const uint32_t c = lane + q * G;
+                if (c < R) part = part + pp[q] * F.U[0][i0 * R + c];
+            }
+            const double m = group_sum<G>(part);
+            const double x = val0[i];
+            acc = acc + (x * x - 2.0 * m * x);
+        }
+    }
+    if (lane == 0) red[gi] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int g = 0; g < GPB; g++) s += red[g];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// ||M||^2 from the block partials of every mode's Gram:
+// sum_{a,b} sigma_a sigma_b prod_k G_k[a][b] (common.h:152-226), one block.
+__global__ void k_chain_final(const double* __restrict__ gram_parts, uint32_t nparts, uint32_t N, uint32_t R,
+                              const double* __restrict__ sigma, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    const uint32_t T = R * (R + 1) / 2;
+    double acc = 0.0;
+    for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) {
+        uint32_t a = 0, base = 0;
+        while (base + (R - a) <= e) {
+            base += R - a;
+            a++;
+        }
+        const uint32_t b = a + (e - base);
+        double v = sigma[a] * sigma[b];
+        for (uint32_t k = 0; k < N; k++) {
+            const double* P = gram_parts + (uint64_t)k * nparts * T;
+            double g = 0.0;
+            for (uint32_t p = 0; p < nparts; p++) g += P[(uint64_t)p * T + e];
+            v = v * g;
+        }
+        acc += a == b ? v : 2.0 * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (uint32_t t = 0; t < blockDim.x; t++) s += red[t];
+        out[0] = s;
+    }
+}
+
+}  // namespace ftl

@@ -13,6 +13,9 @@
 
 namespace ftl {
 
+// most modes a sampler or sparse tensor handles
+constexpr int kMaxModes = 16;
+
 // ---------------------------------------------------------------------------
 // status handling: every entry point returns an ftl_status and records the
 // cause in a thread-local string (the reference exits instead,

@@ -1,0 +1,216 @@
+// jacobi.hpp -- block-level cyclic Jacobi eigensolver for small symmetric
+// matrices (R <= 128), used by computeM and by the sampled-Gram pseudo-inverse
+// of the ALS step.
+#pragma once
+
+#include "ftl_common.hpp"
+
+namespace ftl {
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi eigensolver for one symmetric R x R matrix held by a block
+// (A in LDS, V in LDS or global scratch).  Round-robin pair order; a round
+// first computes its R/2 rotations from the current A (one thread each),
+// then applies J^T A J in place: each thread owns the 2x2 blocks where a row
+// pair meets a column pair and applies the row rotation, then the column
+// rotation (the same operations, in the same order, as applying all row
+// rotations and then all column rotations); V is rotated in place.  Rotation
+// annihilating A[p][q] (Golub & Van Loan Alg. 8.5.2), skipped when
+// |A[p][q]| <= eps*sqrt(|A[p][p] A[q][q]|).  Products and sums are rounded
+// individually (no contraction).  On return lam is ascending and Vout column
+// v holds the eigenvector of lam[v].
+// ---------------------------------------------------------------------------
+struct JacobiShared {
+    int p[64], q[64], act[64];
+    double c[64], s[64];
+};
+
+// 1/x and 1/sqrt(x): v_rcp_f64 / v_rsq_f64 estimates plus one Newton step.
+__device__ inline double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ inline double fast_rsq(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * r;
+    const double e = fma(-h, r, 0.5);
+    return fma(r, e, r);
+}
+
+// Pair k of round `round` in the circle-method ordering of n2 (even) indices:
+// position 0 fixed, positions 1..n2-1 rotating; returned with p < q.
+__device__ inline void jacobi_pair(int k, int round, int n2, int& p, int& q) {
+    p = k == 0 ? 0 : 1 + ((k - 1 + round) % (n2 - 1));
+    const int kk = n2 - 1 - k;
+    q = kk == 0 ? 0 : 1 + ((kk - 1 + round) % (n2 - 1));
+    if (p > q) {
+        const int t = p;
+        p = q;
+        q = t;
+    }
+}
+
+// Rotation (c, s) annihilating A[p][q]; false (c = 1, s = 0) when the entry
+// is negligible or q is the padding index of an odd size.
+__device__ inline bool jacobi_rotation(const double* A, int LD, int R, int p, int q, double& c, double& s) {
+#pragma clang fp contract(off)
+    c = 1.0;
+    s = 0.0;
+    if (q >= R) return false;
+    const double eps = 2.220446049250313e-16;
+    const double apq = A[p * LD + q], app = A[p * LD + p], aqq = A[q * LD + q];
+    // |apq| <= eps sqrt(|app aqq|), compared in squares (no square root on
+    // the round's critical path)
+    if (apq * apq <= (eps * eps) * fabs(app * aqq) || apq == 0.0) return false;
+    // hardware reciprocal / reciprocal square root, each refined by one
+    // Newton step (error ~1 ulp; the rotation stays orthogonal to working
+    // precision)
+    const double tau = (aqq - app) * fast_rcp(2.0 * apq);
+    double t;
+    if (fabs(tau) > 1e150) {
+        t = 0.5 * fast_rcp(tau);
+    } else {
+        const double x = 1.0 + tau * tau;
+        const double sq = x * fast_rsq(x);
+        t = (tau >= 0 ? 1.0 : -1.0) * fast_rcp(fabs(tau) + sq);
+    }
+    c = fast_rsq(1.0 + t * t);
+    s = t * c;
+    return true;
+}
+
+template <int BS>
+__device__ void jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
+                             double* diag_tmp) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int LD = R + 1;  // odd leading dimension: rows start in different LDS banks
+    const int n2 = R + (R & 1);
+    const int npairs = n2 / 2;
+    for (int e = tid; e < R * R; e += BS) V[(e / R) * LD + e % R] = (e / R == e % R) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 40; sweep++) {
+        int any = 0;
+        for (int round = 0; round + 1 < n2; round++) {
+            // rotations of this round, each computed once from the current A
+            if (tid < npairs) {
+                int p, q;
+                double c, s;
+                jacobi_pair(tid, round, n2, p, q);
+                const bool act = jacobi_rotation(A, LD, R, p, q, c, s);
+                js.p[tid] = p;
+                js.q[tid] = q;
+                js.c[tid] = c;
+                js.s[tid] = s;
+                js.act[tid] = act;
+                any |= act;
+            }
+            __syncthreads();
+            // J^T A J in place: the 2x2 block where row pair k meets column
+            // pair k2 belongs to one thread, which applies the row rotation
+            // and then the column rotation (the operation order of applying
+            // all row rotations, then all column rotations)
+            for (int it = tid; it < npairs * npairs; it += BS) {
+                const int k = it / npairs, k2 = it - (it / npairs) * npairs;
+                const bool act1 = js.act[k], act2 = js.act[k2];
+                if (!act1 && !act2) continue;
+                const int p = js.p[k], q = js.q[k], p2 = js.p[k2], q2 = js.q[k2];
+                const double c = js.c[k], s = js.s[k], c2 = js.c[k2], s2 = js.s[k2];
+                // an index of value R (padding of an odd size) only occurs in
+                // inactive pairs; its entries are treated as absent
+                const bool qv = q < R, q2v = q2 < R;
+                const double app2 = A[p * LD + p2];
+                const double apq2 = q2v ? A[p * LD + q2] : 0.0;
+                const double aqp2 = qv ? A[q * LD + p2] : 0.0;
+                const double aqq2 = qv && q2v ? A[q * LD + q2] : 0.0;
+                double rpp2 = app2, rpq2 = apq2, rqp2 = aqp2, rqq2 = aqq2;
+                if (act1) {
+                    rpp2 = c * app2 - s * aqp2;
+                    rqp2 = s * app2 + c * aqp2;
+                    rpq2 = c * apq2 - s * aqq2;
+                    rqq2 = s * apq2 + c * aqq2;
+                }
+                double npp2 = rpp2, npq2 = rpq2, nqp2 = rqp2, nqq2 = rqq2;
+                if (act2) {
+                    npp2 = c2 * rpp2 - s2 * rpq2;
+                    npq2 = s2 * rpp2 + c2 * rpq2;
+                    nqp2 = c2 * rqp2 - s2 * rqq2;
+                    nqq2 = s2 * rqp2 + c2 * rqq2;
+                }
+                A[p * LD + p2] = npp2;
+                if (q2v) A[p * LD + q2] = npq2;
+                if (qv) A[q * LD + p2] = nqp2;
+                if (qv && q2v) A[q * LD + q2] = nqq2;
+            }
+            // V J
+            for (int it = tid; it < R * npairs; it += BS) {
+                const int i = it / npairs, k2 = it - (it / npairs) * npairs;
+                if (!js.act[k2]) continue;
+                const int p = js.p[k2], q = js.q[k2];
+                const double c = js.c[k2], s = js.s[k2];
+                const double vp = V[i * LD + p], vq = V[i * LD + q];
+                V[i * LD + p] = c * vp - s * vq;
+                V[i * LD + q] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+        if (!__syncthreads_or(any)) break;
+        // converged when every pair meets the skip test: the next sweep would
+        // rotate nothing, so stopping here gives the same result
+        int open = 0;
+        for (int w = tid; w < R * R; w += BS) {
+            const int p = w / R, q = w - (w / R) * R;
+            if (p < q) {
+                const double apq = A[p * LD + q];
+                const double thr = (2.220446049250313e-16 * 2.220446049250313e-16) * fabs(A[p * LD + p] * A[q * LD + q]);
+                open |= !(apq * apq <= thr || apq == 0.0);
+            }
+        }
+        if (!__syncthreads_or(open)) break;
+    }
+    // stable ascending sort of the diagonal; Vout (R x R, dense) column v is
+    // the eigenvector of lam[v]
+    for (int v = tid; v < R; v += BS) diag_tmp[v] = A[v * LD + v];
+    __syncthreads();
+    for (int w = tid; w < R * R; w += BS) {
+        const int v = w / R, i = w - v * R;
+        const double d = diag_tmp[v];
+        int rank = 0;
+        for (int u = 0; u < R; u++) {
+            const double du = diag_tmp[u];
+            rank += (du < d) || (du == d && u < v);
+        }
+        if (i == 0) lam[rank] = d;
+        Vout[i * R + rank] = V[i * LD + v];
+    }
+    __syncthreads();
+}
+
+// Jacobi storage of a block: A ((R+1) R doubles, rows padded to an odd
+// length) always in LDS; V ((R+1) R) and the sorted eigenvectors (R^2) in LDS
+// as well when all three fit in kJacobiLdsDoubles (R <= 64), otherwise in the
+// block's global scratch (R = 128).
+constexpr int kJacobiLdsDoubles = 3 * 65 * 64;
+constexpr int kJacobiThreads = 1024;
+
+struct JacobiStore {
+    double *A, *V, *vecs;
+};
+
+__device__ inline JacobiStore jacobi_store(double* smem, double* work, int R) {
+    const int LD = R + 1;
+    JacobiStore st;
+    st.A = smem;
+    if (3 * LD * R <= kJacobiLdsDoubles) {
+        st.V = smem + LD * R;
+        st.vecs = smem + 2 * LD * R;
+    } else {
+        st.V = work;
+        st.vecs = work + LD * R;
+    }
+    return st;
+}
+
+}  // namespace ftl

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ftl_common.hpp"
+#include "sampler_internal.hpp"
 #include "sampler_kernels.hpp"
 
 namespace ftl {
@@ -148,7 +149,8 @@ struct Sampler {
         m.n64 = std::min<uint64_t>(full, m.sh.node_count);
     }
 
-    void build_mode(uint32_t k) {
+    void build_mode(uint32_t k, hipStream_t st = nullptr) {
+        if (!st) st = stream;
         ModeHost& m = modes[k];
         BuildArgs ba;
         ba.U = m.U64;
@@ -158,13 +160,13 @@ struct Sampler {
         ba.ts64 = ts64;
         ba.first_leaf = m.sh.node_count - m.sh.leaf_count;
         const size_t lds = (size_t)m.F * R * sizeof(double);
-        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)m.sh.leaf_count), dim3(256), lds, stream, ba);
+        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)m.sh.leaf_count), dim3(256), lds, st, ba);
         FTL_HIP_CHECK(hipGetLastError());
         int64_t start = (int64_t)m.sh.nodes_before_lfill, end = (int64_t)ba.first_leaf;
         for (int64_t level = (int64_t)m.sh.lfill_level; level >= 0; level--) {
             if (end > start) {
                 const uint64_t work = (uint64_t)(end - start) * ts64;
-                hipLaunchKernelGGL(k_level_sum, dim3(grid_for(work, 256)), dim3(256), 0, stream, m.tree64,
+                hipLaunchKernelGGL(k_level_sum, dim3(grid_for(work, 256)), dim3(256), 0, st, m.tree64,
                                    (uint64_t)start, (uint64_t)end, ts64);
                 FTL_HIP_CHECK(hipGetLastError());
             }
@@ -173,11 +175,11 @@ struct Sampler {
         }
         if (precision == FTL_FP32) {
             const uint64_t nel = m.I * R;
-            hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, stream, m.U64, m.U32, nel);
+            hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, st, m.U64, m.U32, nel);
             FTL_HIP_CHECK(hipGetLastError());
             if (m.tree32) {
                 const uint64_t work = (m.sh.node_count - m.n64) * ts32;
-                hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(work, 256)), dim3(256), 0, stream, m.tree64,
+                hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(work, 256)), dim3(256), 0, st, m.tree64,
                                    m.tree32, m.n64, m.sh.node_count, ts64, ts32);
                 FTL_HIP_CHECK(hipGetLastError());
             }
@@ -229,8 +231,10 @@ struct Sampler {
         throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
     }
 
+    // alias: U are device arrays that the sampler uses in place as its FP64
+    // factors (neither copied nor freed; the owner keeps them alive)
     void init(uint32_t N_, const uint64_t* I, uint64_t J_, uint64_t R_, const double* const* U, bool on_device,
-              int prec) {
+              int prec, bool alias = false) {
         FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
         FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
         FTL_REQUIRE(J_ >= 1 && J_ <= (1ull << 40), "J must be in [1, 2^40]");
@@ -270,9 +274,13 @@ struct Sampler {
             m.I = I[k];
             m.F = std::min<uint64_t>(R, I[k]);
             set_mode_precision(m);
-            m.U64 = dalloc<double>(m.I * R);
-            FTL_HIP_CHECK(hipMemcpyAsync(m.U64, U[k], m.I * R * sizeof(double),
-                                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+            if (alias) {
+                m.U64 = const_cast<double*>(U[k]);
+            } else {
+                m.U64 = dalloc<double>(m.I * R);
+                FTL_HIP_CHECK(hipMemcpyAsync(m.U64, U[k], m.I * R * sizeof(double),
+                                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+            }
             m.tree64 = dalloc<double>(m.sh.node_count * ts64 + pad);
             FTL_HIP_CHECK(hipMemsetAsync(m.tree64 + m.sh.node_count * ts64, 0, pad * sizeof(double), stream));
             if (precision == FTL_FP32) {
@@ -492,6 +500,34 @@ struct Sampler {
 struct ftl_sampler {
     ftl::Sampler impl;
 };
+
+namespace ftl {
+
+Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,
+                             int precision) {
+    std::unique_ptr<Sampler> s(new Sampler());
+    s->init(N, I, J, R, U_dev, true, precision, true);
+    return s.release();
+}
+
+void sampler_delete(Sampler* s) { delete s; }
+
+void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* w, const double* draws,
+                  hipStream_t st) {
+    s->draw_device(j, samples, h, w, draws, st);
+}
+
+void sampler_rebuild(Sampler* s, uint32_t j, hipStream_t st) {
+    FTL_REQUIRE(j < s->N, "update_sampler: mode index out of range");
+    s->build_mode(j, st);
+}
+
+void sampler_set_seed(Sampler* s, uint64_t seed, uint64_t counter) {
+    s->seed = seed;
+    s->call = counter;
+}
+
+}  // namespace ftl
 
 using ftl::guarded;
 

@@ -1,0 +1,27 @@
+// sampler_internal.hpp -- the part of the sampler that the ALS step drives
+// from another translation unit: a sampler whose FP64 factors are device
+// arrays owned by the caller (the ALS factor matrices, updated in place, as
+// the reference's sampler aliases cp_decomp.U, als.hpp:39-55), its draw and
+// its tree rebuild, all enqueued on a caller-chosen stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ftl {
+
+struct Sampler;
+
+Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,
+                             int precision);
+void sampler_delete(Sampler* s);
+// KRPDrawSamples(j) into device buffers (samples N x J, h J x R, w J); draws
+// NULL -> the built-in Philox uniforms of the next call
+void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* w, const double* draws,
+                  hipStream_t st);
+// update_sampler(j): rebuild tree j from the (aliased) factor j
+void sampler_rebuild(Sampler* s, uint32_t j, hipStream_t st);
+void sampler_set_seed(Sampler* s, uint64_t seed, uint64_t counter);
+
+}  // namespace ftl

@@ -137,6 +137,138 @@ int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value);
 int ftl_sampler_enable_timing(ftl_sampler* s, int enable);
 int ftl_sampler_last_timing(ftl_sampler* s, double* out4);
 
+
+/* ------------------------------------------------------------------------
+ * Sparse tensor and sampled CP-ALS
+ * ------------------------------------------------------------------------ */
+typedef struct ftl_sparse ftl_sparse;
+typedef struct ftl_als ftl_als;
+
+/*
+ * SparseTensor(idx, values, "sort") -- replaces the SparseTensor ctor
+ * (cpp_ext/sparse_tensor.hpp:46-77) and its per-mode SortIdxLookup ctor
+ * (cpp_ext/sort_lookup.hpp:33-60), bound at cpp_ext/als_module.cpp:100-107.
+ * idx: nnz x N uint32 row-major, values: nnz FP64 (host).  dims: NULL for
+ * max coordinate + 1 per mode (the reference takes the sizes from the file,
+ * tensors.py:85-93), else the N mode sizes.  For every mode j the nonzeros
+ * are sorted in HBM by their coordinates on the other modes (ties keep input
+ * order).  Limits: nnz < 2^32; per mode, the bit widths of the other modes'
+ * sizes sum to <= 64 (FTL_EUNSUPPORTED otherwise).
+ */
+int ftl_sparse_create(uint32_t N, uint64_t nnz, const uint32_t* idx_host, const double* values_host,
+                      const uint64_t* dims, ftl_sparse** out);
+
+/* Same from device arrays (read during the call, not retained). */
+int ftl_sparse_create_device(uint32_t N, uint64_t nnz, const uint32_t* idx_dev, const double* values_dev,
+                             const uint64_t* dims, ftl_sparse** out);
+
+int ftl_sparse_destroy(ftl_sparse* X);
+
+/* out[0] = N, out[1] = nnz, out[2 + k] = size of mode k. */
+int ftl_sparse_info(ftl_sparse* X, uint64_t* out);
+
+/* ||X||^2 (sparse_tensor.hpp:70-74; SparseTensor::get_normsq). */
+int ftl_sparse_normsq(ftl_sparse* X, double* out);
+
+/*
+ * execute_downsampled_mttkrp_py(samples, lhs, j, result) -- replaces
+ * SparseTensor::execute_downsampled_mttkrp (sparse_tensor.hpp:95-115) and
+ * SortIdxLookup::execute_spmm (sort_lookup.hpp:69-132): for each sample s
+ * (Jp x N uint64, cast to uint32) every nonzero equal to it on all modes but
+ * j adds value * lhs[s] (lhs Jp x R) to result row (its mode-j coordinate).
+ * result (result_rows x R, result_rows >= size of mode j) is zero-filled
+ * first.  *found = number of matched nonzeros (found_count).
+ */
+int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* samples_host,
+                      const double* lhs_host, uint64_t R, double* result_host, uint64_t result_rows,
+                      uint64_t* found);
+
+/*
+ * The lookup half of the same call (sort_lookup.hpp:94-103): counts_host[s]
+ * = matched nonzeros of sample s; ids_host (capacity ids_cap) = their input
+ * positions, sample by sample, ascending within a sample.  Test surface.
+ */
+int ftl_sparse_lookup(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* samples_host,
+                      uint64_t* counts_host, uint64_t* ids_host, uint64_t ids_cap, uint64_t* found);
+
+/*
+ * compute_residual_normsq(sigma, U) -- replaces
+ * SortIdxLookup::compute_residual_normsq (sort_lookup.hpp:134-197) with
+ * ATB_chain_prod_sum (common.h:215-226), reached through
+ * SparseTensor::compute_residual_normsq (sparse_tensor.hpp:117-119):
+ * ||X||^2 - 2 <X, M> + ||M||^2 with M = [[sigma; U]].  U_host[k] is
+ * I[k] x R FP64 (host), I[k] >= size of mode k.
+ */
+int ftl_sparse_residual_normsq(ftl_sparse* X, uint64_t R, const uint64_t* I, const double* sigma_host,
+                               const double* const* U_host, double* out);
+
+/*
+ * ALS(LowRankTensor(R, U), X) -- replaces the LowRankTensor ctor
+ * (low_rank_tensor.hpp:52-68, sigma = 1) and the ALS ctor (als.hpp:31-37),
+ * bound at als_module.cpp:85-98, 111-115.  Copies the factors U_host[k]
+ * (I[k] x R FP64) to HBM; they stay there and are updated in place.  X may be
+ * NULL for a factor-only handle (renormalize / normsq only).
+ */
+int ftl_als_create(ftl_sparse* X, uint32_t N, const uint64_t* I, uint64_t R, const double* const* U_host,
+                   ftl_als** out);
+int ftl_als_destroy(ftl_als* a);
+
+/* initialize_ds_als(J, "efficient") (als.hpp:39-55): the leverage sampler on
+ * the handle's device factors (shared, not copied). */
+int ftl_als_initialize_ds(ftl_als* a, uint64_t J, int precision);
+
+/* Seed and counter of the sampler's built-in uniforms. */
+int ftl_als_set_seed(ftl_als* a, uint64_t seed, uint64_t counter);
+
+/*
+ * execute_ds_als_update(j, renormalize, update_sampler) -- replaces
+ * ALS::execute_ds_als_update (als.hpp:57-220): draw J samples excluding mode
+ * j, sort/unique them with counts, w' = w c, h' = sqrt(w') h,
+ * pinv(h'^T h') (cut 1e-11, target rank min(J', R)), h'' = sqrt(w') h',
+ * sampled MTTKRP, U_j = MTTKRP pinv, optional renormalize_columns(j) and
+ * update_sampler(j).  draws_host: NULL (built-in uniforms) or 2 N J uniforms
+ * in the layout of efficient_krp_sampler.hpp:221-236.  Returns when done.
+ */
+int ftl_als_ds_update(ftl_als* a, uint32_t j, int renormalize, int update_sampler, const double* draws_host);
+
+/* Same with the uniforms in HBM (or NULL), enqueued on ftl_als_stream(a);
+ * waits once inside for the match count.  Returns before the tail finishes. */
+int ftl_als_ds_update_device(ftl_als* a, uint32_t j, int renormalize, int update_sampler,
+                             const double* draws_dev);
+
+/* Factor k to / from host memory (I[k] x R FP64).  set_factor does not
+ * rebuild the sampler's tree: call ftl_als_update_sampler (update_sampler,
+ * efficient_krp_sampler.hpp:73-75). */
+int ftl_als_get_factor(ftl_als* a, uint32_t k, double* U_host);
+int ftl_als_set_factor(ftl_als* a, uint32_t k, const double* U_host);
+int ftl_als_update_sampler(ftl_als* a, uint32_t k);
+
+/* get_sigma / set_sigma (low_rank_tensor.hpp:181-193). */
+int ftl_als_get_sigma(ftl_als* a, double* sigma_host);
+int ftl_als_set_sigma(ftl_als* a, const double* sigma_host);
+
+/* renormalize_columns(j), j = -1 for every mode (low_rank_tensor.hpp:203-260). */
+int ftl_als_renormalize_columns(ftl_als* a, int j);
+
+/* X.compute_residual_normsq(sigma, U) with the handle's device factors. */
+int ftl_als_residual_normsq(ftl_als* a, double* out);
+
+/* LowRankTensor::get_normsq (low_rank_tensor.hpp:71-78): ||[[sigma; U]]||^2. */
+int ftl_als_lowrank_normsq(ftl_als* a, double* out);
+
+/* Device address of factor k (I[k] x R FP64) for device-resident callers. */
+int ftl_als_factor_device(ftl_als* a, uint32_t k, double** out);
+
+/* Last update: out[0] = unique samples J', out[1] = matched nonzeros
+ * (found_count), out[2] = touched rows of U_j, out[3] = reduction pieces;
+ * with timing enabled, ms of: out[4] draw, out[5] dedup + pinv + probe,
+ * out[6] MTTKRP + solve, out[7] renormalize + tree rebuild. */
+int ftl_als_last_stats(ftl_als* a, double* out8);
+int ftl_als_enable_timing(ftl_als* a, int enable);
+
+/* Stream of the handle (hipStream_t). */
+void* ftl_als_stream(ftl_als* a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,13 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_covers_header():
     from fast_tensor_leverage_amd import _lib
-    bound = set(_lib.SIGNATURES)
-    try:
-        from fast_tensor_leverage_amd import _als_sigs
-        bound |= set(_als_sigs.SIGNATURES)
-    except ImportError:
-        pass
-    assert _declared() == bound
+    assert _declared() == set(_lib.SIGNATURES)
 
 
 def test_no_device_is_reported_not_faked():
