@@ -93,6 +93,228 @@ def cpu_baseline(U, draws, J, j, gpu_samples, sample_j):
     return rec, cmp
 
 
+# ---------------------------------------------------------------------------
+# sampled CP-ALS workload (--workload als)
+# ---------------------------------------------------------------------------
+ALS_SIZES = {
+    # dims, nnz, R, J, sweeps
+    "c5": ((2 ** 22, 2 ** 21, 2 ** 17), 200_000_000, 25, 2 ** 16, 10),
+    "small": ((2 ** 16, 2 ** 15, 2 ** 12), 2_000_000, 25, 2 ** 16, 10),
+}
+
+
+def zipf_coords(torch, n, size, s, gen, dev):
+    """Truncated Zipf(s) indices in [0, n) by inverse transform of the
+    continuous density x^-s on [1, n+1)."""
+    u = torch.rand(size, generator=gen, device=dev, dtype=torch.float64)
+    a = 1.0 - (n + 1.0) ** (1.0 - s)
+    x = (1.0 - u * a) ** (1.0 / (1.0 - s))
+    return (torch.floor(x).to(torch.int64) - 1).clamp_(0, n - 1)
+
+
+def make_sparse_c5(torch, dims, nnz, R, dev, seed=2024):
+    """Distinct Zipf(1.1) coordinates (int32), values = rank-R CP model with
+    N(0,1) factors + 1% Gaussian noise (FP64), generated on the device."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    b1, b2 = (dims[1] - 1).bit_length(), (dims[2] - 1).bit_length()
+    keys = torch.empty(0, dtype=torch.int64, device=dev)
+    while keys.numel() < nnz:
+        m = max(int(1.25 * (nnz - keys.numel())), 1_000_000)
+        k = (zipf_coords(torch, dims[0], m, 1.1, gen, dev) << (b1 + b2)) | \
+            (zipf_coords(torch, dims[1], m, 1.1, gen, dev) << b2) | zipf_coords(torch, dims[2], m, 1.1, gen, dev)
+        keys = torch.unique(torch.cat([keys, k]))
+        del k
+    keys = keys[torch.randperm(keys.numel(), generator=gen, device=dev)[:nnz]]
+    idx = torch.stack([keys >> (b1 + b2), (keys >> b2) & ((1 << b1) - 1), keys & ((1 << b2) - 1)], dim=1)
+    del keys
+    idx = idx.to(torch.int32).contiguous()
+    A = [torch.randn((d, R), generator=gen, device=dev, dtype=torch.float64) for d in dims]
+    vals = torch.empty(nnz, dtype=torch.float64, device=dev)
+    step = 10_000_000
+    for s0 in range(0, nnz, step):
+        s1 = min(nnz, s0 + step)
+        p = A[0][idx[s0:s1, 0].long()]
+        for k in range(1, 3):
+            p *= A[k][idx[s0:s1, k].long()]
+        vals[s0:s1] = p.sum(dim=1)
+    vals += 0.01 * vals.std() * torch.randn(nnz, generator=gen, device=dev, dtype=torch.float64)
+    del A
+    return idx, vals
+
+
+def als_main(args):
+    import ctypes
+    import torch
+    sys.path.insert(0, HERE)
+    import fast_tensor_leverage_amd as ftl
+    from fast_tensor_leverage_amd import _lib
+    dims, nnz, R, J, sweeps = ALS_SIZES[args.als_size]
+    if args.sweeps:
+        sweeps = args.sweeps
+    dev = torch.device("cuda", 0)
+    N = len(dims)
+    log(f"ALS workload {args.als_size}: dims={dims} nnz={nnz} R={R} J={J} sweeps={sweeps}")
+    t0 = time.perf_counter()
+    idx, vals = make_sparse_c5(torch, dims, nnz, R, dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    log(f"generated {nnz} nonzeros in {t_gen:.1f} s")
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    dims_arr = np.array(dims, dtype=np.uint64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.ftl_sparse_create_device(N, nnz, idx.data_ptr(), vals.data_ptr(), dims_arr.ctypes.data,
+                                          ctypes.byref(h)))
+    t_build = time.perf_counter() - t0
+    log(f"sparse tensor built in {t_build:.2f} s")
+    X = ftl.SparseTensor.__new__(ftl.SparseTensor)
+    X._h, X.N, X.nnz, X.dims, X.last_found = h, N, nnz, list(dims), 0
+    host_idx = host_vals = None
+    if args.als_oracle:
+        host_idx = idx.cpu().numpy().view(np.uint32).copy()
+        host_vals = vals.cpu().numpy().copy()
+    del idx, vals
+    torch.cuda.empty_cache()
+
+    lhs = ftl.PyLowRank(dims, R, seed=5)
+    U_init = [u.copy() for u in lhs.U]
+    als = ftl.ALS(lhs.ten, X, host_sync=False)
+    lhs.ten.renormalize_columns(-1)
+    sigma0 = lhs.ten.sigma.copy()
+    U_norm = [u.copy() for u in lhs.U]
+    als.initialize_ds_als(J, "efficient", precision=args.als_precision)
+    als.enable_timing(True)
+    rng = np.random.default_rng(99)
+    draws_all = [[rng.random(2 * N * J) for j in range(N)] for _ in range(sweeps)]
+    fits, fit_s, upd_ms, phases, stats = [], [], [], [], []
+
+    def fit():
+        t = time.perf_counter()
+        f = 1.0 - np.sqrt(als.residual_normsq()) / np.sqrt(X.get_normsq())
+        fit_s.append(time.perf_counter() - t)
+        return f
+
+    fits.append(fit())
+    log(f"initial fit {fits[-1]:.6f} ({fit_s[-1] * 1e3:.1f} ms)")
+    t_all = time.perf_counter()
+    for sw in range(sweeps):
+        for j in range(N):
+            t = time.perf_counter()
+            als.execute_ds_als_update(j, True, True, random_draws=draws_all[sw][j])
+            upd_ms.append((time.perf_counter() - t) * 1e3)
+            st = als.last_stats()
+            stats.append(st)
+        if (sw + 1) % 5 == 0 or sw + 1 == sweeps:
+            fits.append(fit())
+        log(f"sweep {sw + 1}: last update {upd_ms[-1]:.2f} ms, found {stats[-1]['found']}, fit {fits[-1]:.6f}")
+    t_total = time.perf_counter() - t_all
+    upd = np.array(upd_ms)
+    ph = {k: float(np.mean([s[k] for s in stats])) for k in
+          ("draw_ms", "dedup_pinv_probe_ms", "mttkrp_solve_ms", "renormalize_rebuild_ms")}
+    line = {
+        "metric": "sampled CP-ALS (STS-CP) seconds per sweep, sparse Zipf(1.1) tensor",
+        "value": float(upd.sum() / 1e3 / sweeps),
+        "unit": "s/sweep",
+        "n_gpus": 1,
+        "steps": sweeps,
+        "warmup": 0,
+        "ms_per_step": float(upd.sum() / sweeps),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.als_precision == "fp64" else "f64 (FP32 sampler trees)",
+        "data": "synthetic: Zipf(1.1) coordinates, rank-25 CP values + 1% noise, generated on the device",
+        "config": {"workload": f"ALS {args.als_size}: dims={list(dims)} nnz={nnz} R={R} J={J}",
+                   "sweeps": sweeps, "precision": args.als_precision},
+        "update_ms": {"mean": float(upd.mean()), "median": float(np.median(upd)), "max": float(upd.max())},
+        "phase_ms_mean": ph,
+        "unique_samples_mean": float(np.mean([s["unique_samples"] for s in stats])),
+        "matched_nonzeros_mean": float(np.mean([s["found"] for s in stats])),
+        "touched_rows_mean": float(np.mean([s["touched_rows"] for s in stats])),
+        "fits": fits,
+        "fit_eval_s": fit_s,
+        "wall_s_updates_and_fits": t_total,
+        "sparse_construction_s": t_build,
+        "generation_s": t_gen,
+    }
+    if args.als_oracle:
+        line.update(als_oracle_compare(host_idx, host_vals, dims, R, J, U_norm, sigma0, draws_all, fits, sweeps))
+    print(json.dumps(line), flush=True)
+
+
+def als_oracle_compare(idx, vals, dims, R, J, U_norm, sigma0, draws_all, gpu_fits, sweeps):
+    """The same sweeps on the CPU FP64 oracle with identical uniforms and
+    initial factors; returns its fits, their relative difference and the time
+    of one oracle update (the CPU baseline of the ALS step)."""
+    sys.path.insert(0, HERE)
+    from oracle import oracle as orc
+    orc.build()
+    N = len(dims)
+    t0 = time.perf_counter()
+    S = orc.OracleSparse(idx, vals)
+    t_build = time.perf_counter() - t0
+    ref = orc.OracleALS(S, [u.copy() for u in U_norm], J, sigma=sigma0)
+    fits = [ref.fit()]
+    times = []
+    for sw in range(sweeps):
+        for j in range(N):
+            t = time.perf_counter()
+            ref.update(j, draws_all[sw][j])
+            times.append(time.perf_counter() - t)
+        if (sw + 1) % 5 == 0 or sw + 1 == sweeps:
+            fits.append(ref.fit())
+        log(f"oracle sweep {sw + 1}: fit {fits[-1]:.6f}")
+    rel = [abs(a - b) / abs(b) for a, b in zip(gpu_fits, fits)]
+    return {"oracle_fits": fits, "fit_rel_diff_vs_oracle": rel,
+            "cpu_baseline": {"value": float(np.sum(times) / sweeps), "unit": "s/sweep", "cores": orc.num_threads(),
+                             "kind": "port",
+                             "sample": f"the same {sweeps} sweeps on the oracle (sparse lookups built in "
+                                       f"{t_build:.1f} s); host {lscpu_model()}"}}
+
+
+# ---------------------------------------------------------------------------
+# serving stream (--workload serve, and one rate in the default run)
+# ---------------------------------------------------------------------------
+def serve_run(rate, n_requests, budget_gb, slots, precision="fp32", seed=0):
+    sys.path.insert(0, HERE)
+    from fast_tensor_leverage_amd.serve import Server, random_requests, serve_stream
+    reqs = random_requests(n_requests, seed=seed)
+    srv = Server(int(budget_gb * (1 << 30)), slots)
+    # warm-up: one small request (kernel loading)
+    srv.wait(srv.submit([1024, 1024, 1024], 8, 1024, None, 1, precision))
+    out = serve_stream(srv, reqs, rate, seed=seed + 1, precision=precision)
+    lat = out["latency_s"] * 1e3
+    total = sum(q["J"] for q in reqs)
+    st = srv.stats()
+    return {"rate_per_s": rate, "requests": n_requests, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "samples_per_s": total / out["makespan_s"], "makespan_s": out["makespan_s"],
+            "peak_arena_GB": st["peak_bytes"] / 2 ** 30, "budget_GB": budget_gb, "slots": slots,
+            "precision": precision}
+
+
+def serve_main(args):
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    rows = []
+    for r in rates:
+        rows.append(serve_run(r, args.serve_requests, args.serve_budget_gb, args.serve_slots))
+        log(f"rate {r}/s: p50 {rows[-1]['p50_ms']:.2f} ms p99 {rows[-1]['p99_ms']:.2f} ms "
+            f"{rows[-1]['samples_per_s']:,.0f} samples/s")
+    best = max(rows, key=lambda x: x["samples_per_s"])
+    line = {"metric": "KRP draw-request serving: aggregate samples/s; p50/p99 request latency",
+            "value": best["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.serve_requests,
+            "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (FP64 top tree levels, computeM and h in f64)",
+            "data": "synthetic: per-request N(0,1) factors (FP32-rounded) and uniforms from the request seed",
+            "config": {"workload": "serving stream: Poisson arrivals, N in {3,4,5}, I log-uniform [2^10,2^22], "
+                                   "R in {8,16,32,64}, J log-uniform [2^10,2^16]",
+                       "requests": args.serve_requests},
+            "p99_latency_ms": best["p99_ms"], "rates": rows}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +326,30 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
+    ap.add_argument("--workload", choices=["krp", "als", "serve"], default="krp",
+                    help="krp: the KRP leverage-sampling headline (default); als: sampled CP-ALS sweeps; "
+                         "serve: the draw-request stream at several arrival rates")
+    ap.add_argument("--als-size", choices=sorted(ALS_SIZES), default="c5")
+    ap.add_argument("--als-precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--als-oracle", action="store_true", help="rerun the sweeps on the CPU oracle and compare fits")
+    ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=500.0, help="arrival rate of the default run's serving leg")
+    ap.add_argument("--serve-rates", default="50,100,200,500,1000,2000")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-budget-gb", type=float, default=96.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sampler runs on a single MI355X in this build "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "als":
+        als_main(args)
+        return
+    if args.workload == "serve":
+        serve_main(args)
+        return
 
     import torch  # device buffers only; imported before libftl_hip.so so both share one HIP runtime
     sys.path.insert(0, HERE)
@@ -170,6 +411,12 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f)["fp32"]["hbm_bytes_per_launch"]
 
+    serving = None
+    if not args.skip_serve:
+        log(f"serving stream at {args.serve_rate}/s")
+        serving = serve_run(args.serve_rate, args.serve_requests, args.serve_budget_gb, args.serve_slots)
+        log(f"serving: p50 {serving['p50_ms']:.2f} ms p99 {serving['p99_ms']:.2f} ms")
+
     cpu = None
     extra_cmp = {}
     if not args.skip_cpu:
@@ -204,7 +451,8 @@ def main():
         "fp64_roofline_frac": bytes64 / (f64["timing"]["draw_ms"] * 1e-3) / HBM_PEAK,
         "kernel_ms": {"fp32": f32["timing"], "fp64": f64["timing"]},
         "construction_s": {"fp32": f32["build_s"], "fp64": f64["build_s"]},
-        "p99_latency_ms": None,
+        "p99_latency_ms": serving["p99_ms"] if serving else None,
+        "serving": serving,
     }
     line.update(extra_cmp)
     print(json.dumps(line), flush=True)

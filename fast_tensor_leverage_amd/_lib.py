@@ -66,6 +66,15 @@ SIGNATURES = {
     "ftl_als_last_stats": ([_vp, _vp], _i32),
     "ftl_als_enable_timing": ([_vp, _i32], _i32),
     "ftl_als_stream": ([_vp], _vp),
+    # draw-request server
+    "ftl_server_create": ([_u64, _u32, _vp], _i32),
+    "ftl_server_destroy": ([_vp], _i32),
+    "ftl_server_submit": ([_vp, _u32, _vp, _u64, _u64, _u32, _u64, _i32, _vp], _i32),
+    "ftl_server_progress": ([_vp, _vp, _u64, _vp], _i32),
+    "ftl_server_result": ([_vp, _u64, _vp], _i32),
+    "ftl_server_factor": ([_vp, _u64, _u32, _vp], _i32),
+    "ftl_server_cancel": ([_vp, _u64], _i32),
+    "ftl_server_stats": ([_vp, _vp], _i32),
 }
 
 _lib = None

@@ -14,8 +14,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libftl_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["sampler.hip", "als.hip"]
-HEADERS = ["ftl_common.hpp", "sampler_kernels.hpp", "als_kernels.hpp"]
+SOURCES = ["sampler.hip", "als.hip", "serve.hip"]
+HEADERS = ["ftl_common.hpp", "sampler_kernels.hpp", "als_kernels.hpp", "jacobi.hpp", "wave_ops.hpp",
+           "sampler_internal.hpp"]
 
 ARCH = os.environ.get("FTL_OFFLOAD_ARCH", "gfx950")
 
@@ -45,20 +46,37 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build_library(verbose: bool = False) -> str:
-    """Compile every HIP source of the package into one shared library."""
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-I", os.path.join(os.path.dirname(HERE), "include")] + sources() + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
         raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
-    errs = [ln for ln in proc.stderr.splitlines()
-            if "warning" in ln and "packed-fp32-ops" not in ln]
+    errs = [ln for ln in proc.stderr.splitlines() if "warning" in ln and "packed-fp32-ops" not in ln]
     if verbose and errs:
         sys.stderr.write("\n".join(errs) + "\n")
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile every HIP source of the package (one hipcc per source, in
+    parallel) and link them into one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    inc = ["-I", os.path.join(os.path.dirname(HERE), "include")]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    objs = []
+    jobs = []
+    for src in sources():
+        obj = os.path.join(CSRC, os.path.basename(src) + ".o")
+        objs.append(obj)
+        jobs.append([hipcc()] + compile_flags + inc + ["-c", src, "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(len(jobs), 4)) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
+    tmp = LIB_PATH + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
+    for o in objs:
+        os.remove(o)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -105,35 +105,51 @@ struct Sampler {
         pending.clear();
     }
     size_t bytes_held = 0;
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;
+    // optional sub-allocator (a serving arena); hipMalloc / hipFree otherwise
+    DeviceAllocator* allocator = nullptr;
+    bool own_stream = true;
 
     template <typename T>
     T* dalloc(uint64_t count) {
         void* p = nullptr;
         const size_t bytes = (size_t)count * sizeof(T);
-        FTL_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 16));
-        allocs.push_back(p);
+        if (allocator) {
+            p = allocator->alloc(bytes ? bytes : 16);
+            if (!p) throw Error{FTL_ENOMEM, "serving arena exhausted"};
+        } else {
+            FTL_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+        }
+        allocs.push_back({p, bytes ? bytes : 16});
         bytes_held += bytes;
         return (T*)p;
     }
 
+    void release_block(void* p, size_t bytes) {
+        if (allocator) allocator->release(p, bytes);
+        else (void)hipFree(p);
+    }
+
     void dfree(void* p, size_t bytes) {
         if (!p) return;
-        auto it = std::find(allocs.begin(), allocs.end(), p);
-        if (it != allocs.end()) allocs.erase(it);
-        (void)hipFree(p);
-        bytes_held -= bytes;
+        for (size_t i = 0; i < allocs.size(); i++)
+            if (allocs[i].first == p) {
+                release_block(p, allocs[i].second);
+                allocs.erase(allocs.begin() + (long)i);
+                bytes_held -= bytes;
+                return;
+            }
     }
 
     ~Sampler() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : allocs) (void)hipFree(p);
+        for (auto& a : allocs) release_block(a.first, a.second);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& q : pending)
             for (auto e : q) (void)hipEventDestroy(e);
         for (auto e : event_pool) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
     void set_mode_precision(ModeHost& m) {
@@ -233,8 +249,12 @@ struct Sampler {
 
     // alias: U are device arrays that the sampler uses in place as its FP64
     // factors (neither copied nor freed; the owner keeps them alive)
+    // sync_end: wait for the tree builds before returning (false: they stay
+    // queued on the sampler's stream)
+    // use_stream: run on this stream (borrowed, not destroyed) instead of a
+    // stream of the sampler's own
     void init(uint32_t N_, const uint64_t* I, uint64_t J_, uint64_t R_, const double* const* U, bool on_device,
-              int prec, bool alias = false) {
+              int prec, bool alias = false, bool sync_end = true, hipStream_t use_stream = nullptr) {
         FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
         FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
         FTL_REQUIRE(J_ >= 1 && J_ <= (1ull << 40), "J must be in [1, 2^40]");
@@ -247,7 +267,12 @@ struct Sampler {
         ts64 = (uint32_t)tri_stride64(R);
         ts32 = (uint32_t)tri_stride32(R);
         FTL_HIP_CHECK(hipGetDevice(&device));
-        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (use_stream) {
+            stream = use_stream;
+            own_stream = false;
+        } else {
+            FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        }
         static bool attrs_set = false;
         if (!attrs_set) {
             FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_leaf_gram<256>,
@@ -304,7 +329,38 @@ struct Sampler {
         lam = dalloc<double>((uint64_t)(N + 1) * R);
         FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R2 * sizeof(double), stream));
         FTL_HIP_CHECK(hipMemsetAsync(etree, 0, ((uint64_t)N * etree_nodes * ts64 + pad) * sizeof(double), stream));
-        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        if (sync_end) FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // Device bytes init(alias = true) allocates, each block rounded up to
+    // `align` (the factors are the caller's).
+    static size_t footprint(uint32_t N, const uint64_t* I, uint64_t R, int prec, uint32_t fp32_lv, size_t align) {
+        auto up = [&](size_t b) { return (std::max<size_t>(b, 16) + align - 1) / align * align; };
+        const uint64_t ts64 = tri_stride64(R), ts32 = tri_stride32(R);
+        uint64_t rmax = 8;
+        while (rmax < R) rmax *= 2;
+        const uint64_t pad = tri_size(rmax) + 64;
+        size_t total = 0;
+        for (uint32_t k = 0; k < N; k++) {
+            ModeHost m;
+            m.I = I[k];
+            m.F = std::min<uint64_t>(R, I[k]);
+            m.sh = make_shape(m.I, m.F);
+            const uint32_t depth = (uint32_t)m.sh.lfill_level + (m.sh.node_count > m.sh.nodes_upto_lfill ? 1 : 0);
+            uint32_t d64 = depth + 1;
+            if (prec == FTL_FP32) d64 = std::max<uint32_t>(1, depth + 1 > fp32_lv ? depth + 1 - fp32_lv : 0);
+            const uint64_t full = d64 >= 63 ? ~0ull : ((1ull << d64) - 1);
+            const uint64_t n64 = std::min<uint64_t>(full, m.sh.node_count);
+            total += up((m.sh.node_count * ts64 + pad) * 8);
+            if (prec == FTL_FP32) {
+                total += up(m.I * R * 4);
+                if (m.sh.node_count > n64) total += up(((m.sh.node_count - n64) * ts32 + pad) * 4);
+            }
+        }
+        const uint64_t R2 = R * R;
+        total += up(R2 * 8) * 2 + up(N * R2 * 8) + up(((uint64_t)N * (2 * R - 1) * ts64 + pad) * 8) +
+                 up((uint64_t)(N + 1) * 3 * (R + 1) * R * 8) + up((uint64_t)(N + 1) * R * 8);
+        return total;
     }
 
     void update(uint32_t j, const double* Uj, bool on_device) {
@@ -502,6 +558,20 @@ struct ftl_sampler {
 };
 
 namespace ftl {
+
+Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev, int precision,
+                           DeviceAllocator* alloc, hipStream_t st) {
+    std::unique_ptr<Sampler> s(new Sampler());
+    s->allocator = alloc;
+    s->init(N, I, J, R, U_dev, true, precision, true, false, st);
+    return s.release();
+}
+
+hipStream_t sampler_stream(Sampler* s) { return s->stream; }
+
+size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t R, int precision, size_t align) {
+    return Sampler::footprint(N, I, R, precision, kDefaultFp32Levels, align);
+}
 
 Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,
                              int precision) {

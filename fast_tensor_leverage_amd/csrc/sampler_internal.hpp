@@ -13,9 +13,24 @@ namespace ftl {
 
 struct Sampler;
 
+// Device memory source for a sampler (a serving arena); alloc returns null
+// when the request does not fit.
+struct DeviceAllocator {
+    virtual void* alloc(size_t bytes) = 0;
+    virtual void release(void* p, size_t bytes) = 0;
+    virtual ~DeviceAllocator() = default;
+};
+
 Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,
                              int precision);
 void sampler_delete(Sampler* s);
+// Same, allocating from `alloc` and leaving the tree builds queued on stream
+// `st` (borrowed; no host wait).
+Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev, int precision,
+                           DeviceAllocator* alloc, hipStream_t st);
+hipStream_t sampler_stream(Sampler* s);
+// Device bytes sampler_new_async allocates (blocks rounded up to `align`).
+size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t R, int precision, size_t align);
 // KRPDrawSamples(j) into device buffers (samples N x J, h J x R, w J); draws
 // NULL -> the built-in Philox uniforms of the next call
 void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* w, const double* draws,

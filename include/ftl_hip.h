@@ -269,6 +269,47 @@ int ftl_als_enable_timing(ftl_als* a, int enable);
 /* Stream of the handle (hipStream_t). */
 void* ftl_als_stream(ftl_als* a);
 
+/* ------------------------------------------------------------------------
+ * Draw-request server (no counterpart in the reference, whose only surface is
+ * one PySampler per factor set, als_module.cpp:30-82).  Requests for J
+ * samples of the KRP of a factor set (N, I[k], R) are admitted FIFO by HBM
+ * footprint into a device arena of hbm_budget_bytes, run concurrently on up
+ * to max_active streams, and retired as they finish.  Factors are generated
+ * on the device: entries N(0,1) rounded to FP32, from a Philox stream of the
+ * request's seed; the uniforms are the built-in Philox stream of the same
+ * seed.  A result therefore depends only on the request, alone or in any mix.
+ * Not thread-safe: one host thread drives a server.
+ * ------------------------------------------------------------------------ */
+typedef struct ftl_server ftl_server;
+
+int ftl_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, ftl_server** out);
+int ftl_server_destroy(ftl_server* s);
+
+/* Queue a request; j as in KRPDrawSamples (j >= N: every mode).  FTL_ENOMEM
+ * when the request alone exceeds the budget. */
+int ftl_server_submit(ftl_server* s, uint32_t N, const uint64_t* I, uint64_t R, uint64_t J, uint32_t j,
+                      uint64_t seed, int precision, uint64_t* id_out);
+
+/* Retire finished requests and admit waiting ones.  done_ids receives the ids
+ * finished since the last call (*n_done of them; capacity >= max_active plus
+ * the number waiting always suffices). */
+int ftl_server_progress(ftl_server* s, uint64_t* done_ids, uint64_t capacity, uint64_t* n_done);
+
+/* Samples (N x J uint64, row j zero when j < N) of a finished request, then
+ * release it.  FTL_ENOMEM for a request that could not be placed. */
+int ftl_server_result(ftl_server* s, uint64_t id, uint64_t* samples_host);
+
+/* Factor k (I[k] x R FP64) of a finished, unreleased request (test surface). */
+int ftl_server_factor(ftl_server* s, uint64_t id, uint32_t k, double* U_host);
+
+/* Drop a request: waiting ones at once, running ones when their queued work
+ * ends, finished ones with their result. */
+int ftl_server_cancel(ftl_server* s, uint64_t id);
+
+/* out[0..7] = active, waiting, finished-unreleased, arena bytes in use, peak
+ * bytes in use, admitted, completed, cancelled. */
+int ftl_server_stats(ftl_server* s, double* out8);
+
 #ifdef __cplusplus
 }
 #endif

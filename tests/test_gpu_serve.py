@@ -1,0 +1,125 @@
+"""The draw-request server: a served request equals the one-shot sampler on
+the same factors and uniforms (bitwise), a request's result is the same
+served alone and in a mix, admission respects the HBM budget, and cancelled
+requests leave the others unchanged."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    {"dims": [3000, 2000, 1500], "R": 16, "J": 4096, "seed": 11},
+    {"dims": [70000, 512, 1024, 900], "R": 32, "J": 20000, "seed": 12},
+    {"dims": [1024, 1100, 1300, 1500, 1700], "R": 8, "J": 1024, "seed": 13},
+    {"dims": [200000, 3000, 5000], "R": 64, "J": 8192, "seed": 14},
+    {"dims": [50000, 60000, 70000], "R": 32, "J": 65536, "seed": 15},
+    {"dims": [5000, 4000, 3000, 2000], "R": 16, "J": 3000, "seed": 16},
+]
+
+
+@pytest.fixture(scope="module")
+def ftl():
+    import fast_tensor_leverage_amd as ftl
+    return ftl
+
+
+def _run_alone(ftl, q, precision):
+    srv = ftl.Server(8 << 30, 4)
+    rid = srv.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)
+    return srv.wait(rid)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_served_request_equals_sampler(ftl, precision):
+    print(f"[gpu] serve vs sampler precision={precision}", flush=True)
+    q = MIX[1]
+    srv = ftl.Server(8 << 30, 4)
+    rid = srv.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)
+    while rid not in srv.progress():
+        pass
+    U = [srv.factor(rid, k) for k in range(len(q["dims"]))]
+    got = srv.result(rid)
+    for u in U:  # N(0,1) entries rounded to FP32
+        assert np.array_equal(u, u.astype(np.float32).astype(np.float64))
+        assert abs(u.mean()) < 0.05 and abs(u.std() - 1.0) < 0.05
+    s = ftl.Sampler(U, q["J"], q["R"], "efficient", precision=precision)
+    s.set_seed(q["seed"], 0)
+    ref = np.zeros((len(U), q["J"]), dtype=np.uint64)
+    s.KRPDrawSamples(len(U) + 1, ref)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_alone_equals_mixed(ftl, precision):
+    print(f"[gpu] alone vs mixed precision={precision}", flush=True)
+    alone = {i: _run_alone(ftl, q, precision) for i, q in enumerate(MIX)}
+    srv = ftl.Server(64 << 30, 4)
+    ids = {}
+    got = {}
+    # staggered submission: later requests arrive while earlier ones run
+    for i, q in enumerate(MIX):
+        ids[srv.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)] = i
+        for rid in srv.progress():
+            got[ids[rid]] = srv.result(rid)
+    while len(got) < len(MIX):
+        for rid in srv.progress():
+            got[ids[rid]] = srv.result(rid)
+    st = srv.stats()
+    assert st["completed"] == len(MIX) and st["bytes_in_use"] == 0
+    for i in range(len(MIX)):
+        np.testing.assert_array_equal(got[i], alone[i], err_msg=f"request {i}")
+
+
+def test_admission_by_budget(ftl):
+    """With a budget for about two requests the arena never exceeds it and
+    every request completes."""
+    q = {"dims": [100000, 100000, 100000], "R": 32, "J": 4096}
+    srv = ftl.Server(1 << 30, 8)
+    rid0 = srv.submit(q["dims"], q["R"], q["J"], None, 1)
+    st = srv.stats()
+    one = st["bytes_in_use"]
+    assert st["active"] == 1 and one > 0
+    del srv
+    budget = int(2.5 * one)
+    srv = ftl.Server(budget, 8)
+    ids = [srv.submit(q["dims"], q["R"], q["J"], None, 100 + i) for i in range(6)]
+    st = srv.stats()
+    assert st["active"] == 2 and st["waiting"] == 4
+    done = set()
+    while len(done) < len(ids):
+        for rid in srv.progress():
+            srv.result(rid)
+            done.add(rid)
+        assert srv.stats()["peak_bytes"] <= budget
+    assert done == set(ids)
+    with pytest.raises(MemoryError):
+        srv.submit([10 ** 7, 10 ** 7, 10 ** 7], 64, 1024, None, 5)
+    del rid0
+
+
+def test_cancel_leaves_others_unchanged(ftl):
+    alone = {i: _run_alone(ftl, q, "fp32") for i, q in enumerate(MIX[:4])}
+    srv = ftl.Server(64 << 30, 2)
+    ids = {srv.submit(q["dims"], q["R"], q["J"], None, q["seed"], "fp32"): i for i, q in enumerate(MIX[:4])}
+    victim = {v: k for k, v in ids.items()}
+    srv.cancel(victim[3])  # still waiting (two slots)
+    srv.cancel(victim[0])  # running
+    got = {}
+    while len(got) < 2:
+        for rid in srv.progress():
+            got[ids[rid]] = srv.result(rid)
+    assert set(got) == {1, 2}
+    for i in got:
+        np.testing.assert_array_equal(got[i], alone[i])
+    st = srv.stats()
+    assert st["cancelled"] == 2 and st["active"] == 0
+
+
+def test_bad_requests(ftl):
+    srv = ftl.Server(1 << 30, 2)
+    with pytest.raises(ValueError):
+        srv.submit([100, 100], 0, 10)
+    with pytest.raises(ValueError):
+        srv.submit([100, 100], 8, 10, precision="fp16")
+    with pytest.raises(ValueError):
+        srv.result(12345)
