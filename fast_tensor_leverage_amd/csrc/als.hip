@@ -301,7 +301,7 @@ struct SparseDev {
         double* part = stage.alloc<double>(1024 + 1);
         hipLaunchKernelGGL(k_sumsq_partial<256>, dim3(1024), dim3(256), 0, stream, d_val, nnz, part);
         FTL_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, stream, part, 1024u, part + 1024);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, stream, part, 1024u, part + 1024);
         FTL_LAUNCH_CHECK();
         FTL_HIP_CHECK(hipMemcpyAsync(&normsq, part + 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
@@ -347,28 +347,33 @@ void probe_and_count(SparseDev& X, uint32_t j, const uint32_t* tup, const uint32
     FTL_LAUNCH_CHECK();
 }
 
+// Returns the block count of the row-finalise kernel (the number of column
+// partials written to colsq when it is given).
 template <int G, int CPL>
-void launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
-                   const double* pinv, double* out, hipStream_t st) {
+unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
+                       const double* pinv, double* out, double* colsq, hipStream_t st) {
     constexpr int GPB = 256 / G;
     const unsigned blocks = grid_for(T, GPB, 16384);
     hipLaunchKernelGGL((k_item_partial<G, CPL>), dim3(blocks), dim3(256), 0, st, m.seg_start, m.piece_off,
                        m.counters, m.spay, X.val[j], lhs, R, m.partial);
     FTL_LAUNCH_CHECK();
+    const unsigned fblocks = grid_for(T, GPB, kSegBlocksMax);
     if (pinv)
-        hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(blocks), dim3(256), 0, st, m.seg_row, m.piece_off,
-                           m.counters, m.partial, R, pinv, out);
+        hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(fblocks), dim3(256), 0, st, m.seg_row, m.piece_off,
+                           m.counters, m.partial, R, pinv, out, colsq);
     else
-        hipLaunchKernelGGL((k_seg_final<G, CPL, false>), dim3(blocks), dim3(256), 0, st, m.seg_row, m.piece_off,
-                           m.counters, m.partial, R, pinv, out);
+        hipLaunchKernelGGL((k_seg_final<G, CPL, false>), dim3(fblocks), dim3(256), 0, st, m.seg_row, m.piece_off,
+                           m.counters, m.partial, R, pinv, out, (double*)nullptr);
     FTL_LAUNCH_CHECK();
+    return fblocks;
 }
 
 // Given probed ranges of n samples and T >= 1 matches: list the matches,
 // sort them by output row (stable), form the row segments and their pieces,
 // reduce, and write each touched row of out (times pinv when given).
-void reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
-                    const double* pinv, double* out, Matches& m, Arena& ar, CubTemp& cub, hipStream_t st) {
+unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
+                        const double* pinv, double* out, double* colsq, Matches& m, Arena& ar, CubTemp& cub,
+                        hipStream_t st) {
     m.ensure_t(ar, T, R);
     hipLaunchKernelGGL(k_expand, dim3(grid_for(T, 256)), dim3(256), 0, st, m.off, m.lo, n, T, X.row[j], m.mrow,
                        m.mpay);
@@ -387,9 +392,9 @@ void reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const doub
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, T, m.counters,
                        (int)kCntPieces);
     FTL_LAUNCH_CHECK();
-    if (R <= 32) launch_reduce<32, 1>(X, j, m, T, lhs, R, pinv, out, st);
-    else if (R <= 64) launch_reduce<64, 1>(X, j, m, T, lhs, R, pinv, out, st);
-    else launch_reduce<64, 2>(X, j, m, T, lhs, R, pinv, out, st);
+    if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
+    if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
+    return launch_reduce<64, 2>(X, j, m, T, lhs, R, pinv, out, colsq, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +410,7 @@ struct Als {
     uint32_t ibits[kMaxModes] = {};
     double* U[kMaxModes] = {};
     double *sigma = nullptr, *norms = nullptr, *colpart = nullptr, *scal = nullptr, *resid_part = nullptr,
-           *gram_parts = nullptr;
+           *gram_parts = nullptr, *gram_red = nullptr;
     Arena mem;
     CubTemp cub;
     Matches mt;
@@ -446,7 +451,9 @@ struct Als {
         static bool attrs_set = false;
         if (!attrs_set) {
             // A ((R+1) R doubles) in LDS, plus V and the eigenvectors when R <= 64
-            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<kJacobiThreads>,
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<256>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
+            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<1024>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
             attrs_set = true;
         }
@@ -465,10 +472,11 @@ struct Als {
         const uint64_t T = R * (R + 1) / 2;
         sigma = mem.alloc<double>(R);
         norms = mem.alloc<double>(R);
-        colpart = mem.alloc<double>((uint64_t)kColBlocks * R);
+        colpart = mem.alloc<double>((uint64_t)std::max(kColBlocks, kSegBlocksMax) * R);
         scal = mem.alloc<double>(4);
         resid_part = mem.alloc<double>(kResidBlocks);
-        gram_parts = mem.alloc<double>((uint64_t)N * kGramBlocks * T);
+        gram_parts = mem.alloc<double>((uint64_t)N * kGramBlocksMax * T);
+        gram_red = mem.alloc<double>((uint64_t)N * T);
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(128), 0, stream, sigma, R, 1.0);
         FTL_LAUNCH_CHECK();
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
@@ -517,18 +525,19 @@ struct Als {
     }
 
     template <int RMAX>
-    void launch_gram(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts) {
-        hipLaunchKernelGGL((k_gram_partial<256, RMAX>), dim3(kGramBlocks), dim3(256), 0, stream, Xrows, n_dev, n_host,
+    void launch_gram(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts, unsigned nb) {
+        hipLaunchKernelGGL((k_gram_partial<256, RMAX>), dim3(nb), dim3(256), 0, stream, Xrows, n_dev, n_host,
                            (uint32_t)R, parts);
         FTL_LAUNCH_CHECK();
     }
 
-    void gram_partials(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts) {
-        if (R <= 8) launch_gram<8>(Xrows, n_dev, n_host, parts);
-        else if (R <= 16) launch_gram<16>(Xrows, n_dev, n_host, parts);
-        else if (R <= 32) launch_gram<32>(Xrows, n_dev, n_host, parts);
-        else if (R <= 64) launch_gram<64>(Xrows, n_dev, n_host, parts);
-        else launch_gram<128>(Xrows, n_dev, n_host, parts);
+    // block partials (nb blocks) of the packed Gram of n rows
+    void gram_partials(const double* Xrows, const uint32_t* n_dev, uint64_t n_host, double* parts, unsigned nb) {
+        if (R <= 8) launch_gram<8>(Xrows, n_dev, n_host, parts, nb);
+        else if (R <= 16) launch_gram<16>(Xrows, n_dev, n_host, parts, nb);
+        else if (R <= 32) launch_gram<32>(Xrows, n_dev, n_host, parts, nb);
+        else if (R <= 64) launch_gram<64>(Xrows, n_dev, n_host, parts, nb);
+        else launch_gram<128>(Xrows, n_dev, n_host, parts, nb);
     }
 
     // execute_ds_als_update(j, renormalize, update_sampler) (als.hpp:57-220)
@@ -556,10 +565,14 @@ struct Als {
         hipLaunchKernelGGL(k_dedup_rows, dim3(grid_for(J * R, 256)), dim3(256), 0, st, ufirst, ustart, mt.counters,
                            J, (uint32_t)R, h, w, hs, hdd);
         FTL_LAUNCH_CHECK();
-        gram_partials(hs, mt.counters + kCntUnique, 0, gpart);
+        gram_partials(hs, mt.counters + kCntUnique, 0, gpart, kGramBlocks);
         const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
-        hipLaunchKernelGGL(k_pinv_partials<kJacobiThreads>, dim3(1), dim3(kJacobiThreads), jac * sizeof(double), st,
-                           gpart, kGramBlocks, mt.counters, (uint32_t)R, pinv, work, lam);
+        if (jacobi_threads_for(R) == 256)
+            hipLaunchKernelGGL(k_pinv_partials<256>, dim3(1), dim3(256), jac * sizeof(double), st, gpart,
+                               kGramBlocks, mt.counters, (uint32_t)R, pinv, work, lam);
+        else
+            hipLaunchKernelGGL(k_pinv_partials<1024>, dim3(1), dim3(1024), jac * sizeof(double), st, gpart,
+                               kGramBlocks, mt.counters, (uint32_t)R, pinv, work, lam);
         FTL_LAUNCH_CHECK();
         // 3. probe the unique tuples on mode j's lookup; the match count
         //    sizes the rest, so it is read back here (one wait per update)
@@ -573,16 +586,17 @@ struct Als {
         record(2);
         // 4. U_j = MTTKRP(h'') pinv on the touched rows, zero elsewhere
         FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), st));
-        if (T > 0) reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, pinv, U[j], mt, mem, cub, st);
+        unsigned colparts = 0;
+        if (T > 0)
+            colparts = reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, pinv, U[j], renormalize ? colpart : nullptr, mt,
+                                      mem, cub, st);
         else FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
         record(3);
         // 5. renormalize (only touched rows are nonzero) and rebuild tree j
         if (renormalize) {
             if (T > 0) {
-                hipLaunchKernelGGL(k_colsq_partial<256>, dim3(kColBlocks), dim3(256), 0, st, U[j], mt.seg_row,
-                                   mt.counters + kCntRows, (uint64_t)0, (uint32_t)R, colpart);
-                FTL_LAUNCH_CHECK();
-                hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(128), 0, st, colpart, kColBlocks, (uint32_t)R, norms,
+                // column sums of squares of the touched rows came with the row finalise
+                hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(256), 0, st, colpart, colparts, (uint32_t)R, norms,
                                    sigma, 0);
                 FTL_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(T * R, 256)), dim3(256), 0, st, U[j], mt.seg_row,
@@ -610,7 +624,7 @@ struct Als {
             hipLaunchKernelGGL(k_colsq_partial<256>, dim3(kColBlocks), dim3(256), 0, st, U[k], nullptr, nullptr, I[k],
                                (uint32_t)R, colpart);
             FTL_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(128), 0, st, colpart, kColBlocks, (uint32_t)R, norms,
+            hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(256), 0, st, colpart, kColBlocks, (uint32_t)R, norms,
                                sigma, j == -1 ? 1 : 0);
             FTL_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(I[k] * R, 256)), dim3(256), 0, st, U[k], nullptr, nullptr,
@@ -623,9 +637,14 @@ struct Als {
     // ||[[sigma; U]]||^2 into scal[1] (enqueued)
     void lowrank_normsq_async() {
         const uint64_t T = R * (R + 1) / 2;
-        for (uint32_t k = 0; k < N; k++) gram_partials(U[k], nullptr, I[k], gram_parts + (uint64_t)k * kGramBlocks * T);
-        hipLaunchKernelGGL(k_chain_final, dim3(1), dim3(256), 0, stream, gram_parts, kGramBlocks, N, (uint32_t)R, sigma,
-                           scal + 1);
+        uint64_t imax = 1;
+        for (uint32_t k = 0; k < N; k++) imax = std::max(imax, I[k]);
+        const unsigned nb = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, (imax + 4095) / 4096));
+        for (uint32_t k = 0; k < N; k++) gram_partials(U[k], nullptr, I[k], gram_parts + (uint64_t)k * nb * T, nb);
+        hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(N * T, 256)), dim3(256), 0, stream, gram_parts, nb,
+                           (uint32_t)T, N, gram_red);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_chain_final, dim3(1), dim3(256), 0, stream, gram_red, N, (uint32_t)R, sigma, scal + 1);
         FTL_LAUNCH_CHECK();
     }
 
@@ -638,9 +657,9 @@ struct Als {
     }
 
     template <int G, int CPL>
-    void launch_residual(const FactorPtrs& F) {
-        hipLaunchKernelGGL((k_residual_partial<G, CPL, 256>), dim3(kResidBlocks), dim3(256), 0, stream, X->keys[0],
-                           X->row[0], X->val[0], X->nnz, X->layout[0], F, sigma, (uint32_t)R, 16u, resid_part);
+    void launch_residual(const FactorPtrs& F, uint32_t s) {
+        hipLaunchKernelGGL((k_residual_partial<G, CPL, 256>), dim3(kResidBlocks), dim3(256), 0, stream, X->keys[s],
+                           X->row[s], X->val[s], X->nnz, X->layout[s], F, sigma, (uint32_t)R, resid_part);
         FTL_LAUNCH_CHECK();
     }
 
@@ -650,10 +669,15 @@ struct Als {
         FTL_REQUIRE(X, "no sparse tensor bound to this handle");
         FactorPtrs F{};
         for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
-        if (R <= 32) launch_residual<32, 1>(F);
-        else if (R <= 64) launch_residual<64, 1>(F);
-        else launch_residual<64, 2>(F);
-        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, stream, resid_part, kResidBlocks, scal);
+        // walk the lookup that leaves out the smallest mode: its factor rows
+        // are the per-nonzero gathers
+        uint32_t s = 0;
+        for (uint32_t k = 1; k < N; k++)
+            if (X->dims[k] < X->dims[s]) s = k;
+        if (R <= 32) launch_residual<32, 1>(F, s);
+        else if (R <= 64) launch_residual<64, 1>(F, s);
+        else launch_residual<64, 2>(F, s);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, stream, resid_part, kResidBlocks, scal);
         FTL_LAUNCH_CHECK();
         lowrank_normsq_async();
         double out[2] = {0, 0};
@@ -770,7 +794,7 @@ int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* sa
         double* res = tmp.alloc<double>(result_rows * R);
         FTL_HIP_CHECK(hipMemcpyAsync(lhs, lhs_host, Jp * R * sizeof(double), hipMemcpyHostToDevice, st));
         FTL_HIP_CHECK(hipMemsetAsync(res, 0, result_rows * R * sizeof(double), st));
-        if (T > 0) reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, nullptr, res, S.mt, S.mem, S.cub, st);
+        if (T > 0) reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, nullptr, res, nullptr, S.mt, S.mem, S.cub, st);
         FTL_HIP_CHECK(hipMemcpyAsync(result_host, res, result_rows * R * sizeof(double), hipMemcpyDeviceToHost, st));
         FTL_HIP_CHECK(hipStreamSynchronize(st));
         if (found) *found = T;

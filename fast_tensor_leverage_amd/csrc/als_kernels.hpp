@@ -36,10 +36,13 @@ namespace ftl {
 
 // Matches of one output row summed sequentially per piece.
 constexpr uint32_t kSegPiece = 256;
-// Blocks of the partial-sum kernels (fixed, so the association is fixed).
-constexpr uint32_t kGramBlocks = 128;
-constexpr uint32_t kColBlocks = 256;
-constexpr uint32_t kResidBlocks = 1024;
+// Blocks of the partial-sum kernels (fixed per launch shape, so the
+// association of every sum is fixed).
+constexpr uint32_t kGramBlocks = 128;      // sampled Gram (J' rows)
+constexpr uint32_t kGramBlocksMax = 512;   // factor Grams (I_k rows)
+constexpr uint32_t kColBlocks = 256;       // column norms over all rows
+constexpr uint32_t kSegBlocksMax = 2048;   // row finalise (+ column norms of touched rows)
+constexpr uint32_t kResidBlocks = 4096;    // fit
 
 // Coordinates of every mode but `skip` packed into one 64-bit key, the
 // lowest-numbered mode in the most significant field, so that key order is
@@ -130,12 +133,21 @@ __global__ void __launch_bounds__(BS) k_sumsq_partial(const double* __restrict__
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void k_sum_final(const double* __restrict__ partial, uint32_t n, double* __restrict__ out) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double s = 0.0;
-        for (uint32_t i = 0; i < n; i++) s += partial[i];
-        out[0] = s;
+// out[0] = sum of partial[0..n): thread t sums t, t+256, ... in order, then
+// a fixed tree over the 256 thread sums.  One block of 256 threads.
+__global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ partial, uint32_t n,
+                                                   double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -412,7 +424,9 @@ __global__ void k_seg_pieces(const uint32_t* __restrict__ seg_start, const uint3
 }
 
 // Piece w of the segmented reduction: the sequential sum over its matches of
-// lhs[u] * value (a group of G lanes, CPL columns per lane).
+// lhs[u] * value (a group of G lanes, CPL columns per lane).  The payloads and
+// values of G matches are fetched at once (one per lane) and broadcast, so the
+// loop only waits for the lhs rows, which are independent across matches.
 template <int G, int CPL>
 __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict__ seg_start,
                                                       const uint32_t* __restrict__ piece_off,
@@ -437,14 +451,23 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
         double acc[CPL];
 #pragma unroll
         for (int q = 0; q < CPL; q++) acc[q] = 0.0;
-        for (uint64_t i = i0; i < i1; i++) {
-            const uint64_t pay = spay[i];
-            const uint64_t u = pay >> 32;
-            const double v = valj[pay & 0xffffffffull];
+        for (uint64_t c0 = i0; c0 < i1; c0 += G) {
+            const int cnt = (int)min<uint64_t>(G, i1 - c0);
+            uint32_t my_u = 0;
+            double my_v = 0.0;
+            if ((int)lane < cnt) {
+                const uint64_t pay = spay[c0 + lane];
+                my_u = (uint32_t)(pay >> 32);
+                my_v = valj[pay & 0xffffffffull];
+            }
+            for (int t = 0; t < cnt; t++) {
+                const uint64_t u = (uint32_t)__shfl(my_u, t, G);
+                const double v = __shfl(my_v, t, G);
 #pragma unroll
-            for (int q = 0; q < CPL; q++) {
-                const uint32_t c = lane + q * G;
-                if (c < R) acc[q] = acc[q] + lhs[u * R + c] * v;
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    if (c < R) acc[q] = acc[q] + lhs[u * R + c] * v;
+                }
             }
         }
 #pragma unroll
@@ -457,18 +480,35 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
 
 // Segment d: sum of its pieces in order, then either the row itself (plain
 // MTTKRP) or its product with pinv (the ALS solve U_j = MTTKRP pinv,
-// als.hpp:189-203), written to row seg_row[d] of out.
+// als.hpp:189-203), written to row seg_row[d] of out.  With colsq given, the
+// block also sums the squares of the rows it wrote per column (block order,
+// then group order: a fixed association) into colsq[block][c], the input of
+// renormalize_columns over the touched rows (all other rows are zero).
 template <int G, int CPL, bool SOLVE>
 __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ seg_row,
                                                    const uint32_t* __restrict__ piece_off,
                                                    const uint32_t* __restrict__ counters,
                                                    const double* __restrict__ partial, uint32_t R,
-                                                   const double* __restrict__ pinv, double* __restrict__ out) {
+                                                   const double* __restrict__ pinv, double* __restrict__ out,
+                                                   double* __restrict__ colsq) {
 #pragma clang fp contract(off)
     constexpr int GPB = 256 / G;
+    constexpr int PINV_LDS = 64 * 64;
     __shared__ double mt_s[GPB][G * CPL];
+    __shared__ double pinv_s[SOLVE ? PINV_LDS : 1];
+    __shared__ double sq_s[GPB][G * CPL];
     const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
+    const bool pinv_in_lds = SOLVE && R * R <= (uint32_t)PINV_LDS;
+    if constexpr (SOLVE) {
+        if (pinv_in_lds)
+            for (uint32_t e = threadIdx.x; e < R * R; e += 256) pinv_s[e] = pinv[e];
+    }
+    const double* P = pinv_in_lds ? pinv_s : pinv;
+    double sq[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; q++) sq[q] = 0.0;
+    __syncthreads();
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
     // every group runs the same number of iterations (block-uniform barrier)
     const uint64_t iters = (D + groups - 1) / groups;
@@ -504,17 +544,29 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
             }
             __syncthreads();
             if (valid) {
+                const uint64_t row = seg_row[d];
 #pragma unroll
                 for (int q = 0; q < CPL; q++) {
                     const uint32_t c = lane + q * G;
                     if (c < R) {
                         double s = 0.0;
-                        for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * pinv[k * R + c];
-                        out[(uint64_t)seg_row[d] * R + c] = s;
+                        for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * P[k * R + c];
+                        out[row * R + c] = s;
+                        sq[q] = sq[q] + s * s;
                     }
                 }
             }
             __syncthreads();
+        }
+    }
+    if (SOLVE && colsq) {
+#pragma unroll
+        for (int q = 0; q < CPL; q++) sq_s[gi][lane + q * G] = sq[q];
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < R; c += 256) {
+            double s = 0.0;
+            for (int g = 0; g < GPB; g++) s += sq_s[g][c];
+            colsq[(uint64_t)blockIdx.x * R + c] = s;
         }
     }
 }
@@ -550,17 +602,24 @@ __global__ void __launch_bounds__(BS) k_colsq_partial(const double* __restrict__
 }
 
 // norms[c] = sqrt(sum of the partials), 1 when <= 1e-7; sigma either the
-// norms (mode_only) or sigma * norms.
-__global__ void k_colnorm_final(const double* __restrict__ partial, uint32_t nparts, uint32_t R,
-                                double* __restrict__ norms, double* __restrict__ sigma, int multiply) {
+// norms or sigma * norms (multiply).  One block of 256 threads: wave w sums
+// columns w, w+4, ...; lane l adds partials l, l+64, ... in order, then a
+// fixed xor butterfly over the 64 lanes.
+__global__ void __launch_bounds__(256) k_colnorm_final(const double* __restrict__ partial, uint32_t nparts,
+                                                       uint32_t R, double* __restrict__ norms,
+                                                       double* __restrict__ sigma, int multiply) {
 #pragma clang fp contract(off)
-    for (uint32_t c = threadIdx.x; c < R; c += blockDim.x) {
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t c = wv; c < R; c += 4) {
         double s = 0.0;
-        for (uint32_t p = 0; p < nparts; p++) s += partial[(uint64_t)p * R + c];
-        double nrm = sqrt(s);
-        if (nrm <= 1e-7) nrm = 1.0;
-        norms[c] = nrm;
-        if (sigma) sigma[c] = multiply ? sigma[c] * nrm : nrm;
+        for (uint32_t p = lane; p < nparts; p += 64) s += partial[(uint64_t)p * R + c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) {
+            double nrm = sqrt(s);
+            if (nrm <= 1e-7) nrm = 1.0;
+            norms[c] = nrm;
+            if (sigma) sigma[c] = multiply ? sigma[c] * nrm : nrm;
+        }
     }
 }
 
@@ -580,69 +639,103 @@ __global__ void k_fill(double* __restrict__ p, uint64_t n, double v) {
 }
 
 // ---------------------------------------------------------------------------
-// residual: sum over nonzeros (mode-0 lookup order) of x^2 - 2 x m, with
-// m = sum_r sigma_r prod_k U_k[i_k][r]; the partial product over modes 1..N-1
-// is reused while consecutive nonzeros share those coordinates
-// (sort_lookup.hpp:134-197).  A group of G lanes walks a run of consecutive
-// nonzeros, lane r holding column r (CPL per lane).
+// residual: sum over nonzeros of x^2 - 2 x m, m = sum_r sigma_r prod_k
+// U_k[i_k][r] (sort_lookup.hpp:134-197), walking the nonzeros in the order of
+// the lookup that leaves out mode s (L.skip): the partial product over the
+// modes != s is reused while consecutive nonzeros share those coordinates,
+// and each nonzero gathers one row of U_s (the caller picks s with the
+// smallest factor, which stays cache resident).  A group of G lanes takes a
+// run of G consecutive nonzeros: their keys, coordinates and values are
+// fetched one per lane and broadcast; lane r holds column r (CPL per lane).
 // ---------------------------------------------------------------------------
+__device__ inline uint64_t shfl_u64(uint64_t v, int src, int width) {
+    const uint32_t lo = __shfl((uint32_t)v, src, width), hi = __shfl((uint32_t)(v >> 32), src, width);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <int G, int CPL, int BS>
-__global__ void __launch_bounds__(BS) k_residual_partial(const uint64_t* __restrict__ keys0,
-                                                         const uint32_t* __restrict__ row0,
-                                                         const double* __restrict__ val0, uint64_t nnz,
+__global__ void __launch_bounds__(BS) k_residual_partial(const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ rows,
+                                                         const double* __restrict__ vals, uint64_t nnz,
                                                          KeyLayout L, FactorPtrs F, const double* __restrict__ sigma,
-                                                         uint32_t R, uint32_t run, double* __restrict__ partial) {
+                                                         uint32_t R, double* __restrict__ partial) {
 #pragma clang fp contract(off)
     constexpr int GPB = BS / G;
     __shared__ double red[GPB];
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
+    const uint32_t s = L.skip;
+    const uint64_t nruns = (nnz + G - 1) / G;
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
     double acc = 0.0;  // identical in every lane of a group
-    for (uint64_t r0 = ((uint64_t)blockIdx.x * GPB + gi) * run; r0 < nnz; r0 += groups * run) {
-        const uint64_t r1 = min<uint64_t>(nnz, r0 + run);
+    for (uint64_t run = (uint64_t)blockIdx.x * GPB + gi; run < nruns; run += groups) {
+        const uint64_t base = run * G;
+        const int cnt = (int)min<uint64_t>(G, nnz - base);
+        uint64_t my_key = 0;
+        uint32_t my_row = 0;
+        double my_val = 0.0;
+        if ((int)lane < cnt) {
+            my_key = keys[base + lane];
+            my_row = rows[base + lane];
+            my_val = vals[base + lane];
+        }
         double pp[CPL];
         uint64_t prev = 0;
-        for (uint64_t i = r0; i < r1; i++) {
-            const uint64_t key = keys0[i];
-            if (i == r0 || key != prev) {
+        for (int t = 0; t < cnt; t++) {
+            const uint64_t key = shfl_u64(my_key, t, G);
+            const uint64_t row = __shfl(my_row, t, G);
+            const double x = __shfl(my_val, t, G);
+            if (t == 0 || key != prev) {
 #pragma unroll
                 for (int q = 0; q < CPL; q++) {
                     const uint32_t c = lane + q * G;
                     double p = 0.0;
                     if (c < R) {
                         p = sigma[c];
-                        for (uint32_t k = 1; k < L.N; k++)
-                            p = p * F.U[k][(uint64_t)key_field(L, key, k) * R + c];
+                        for (uint32_t k = 0; k < L.N; k++)
+                            if (k != s) p = p * F.U[k][(uint64_t)key_field(L, key, k) * R + c];
                     }
                     pp[q] = p;
                 }
                 prev = key;
             }
-            const uint64_t i0 = row0[i];
             double part = 0.0;
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 const uint32_t c = lane + q * G;
-                if (c < R) part = part + pp[q] * F.U[0][i0 * R + c];
+                if (c < R) part = part + pp[q] * F.U[s][row * R + c];
             }
             const double m = group_sum<G>(part);
-            const double x = val0[i];
             acc = acc + (x * x - 2.0 * m * x);
         }
     }
     if (lane == 0) red[gi] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int g = 0; g < GPB; g++) s += red[g];
-        partial[blockIdx.x] = s;
+        double t = 0.0;
+        for (int g = 0; g < GPB; g++) t += red[g];
+        partial[blockIdx.x] = t;
     }
 }
 
-// ||M||^2 from the block partials of every mode's Gram:
-// sum_{a,b} sigma_a sigma_b prod_k G_k[a][b] (common.h:152-226), one block.
-__global__ void k_chain_final(const double* __restrict__ gram_parts, uint32_t nparts, uint32_t N, uint32_t R,
-                              const double* __restrict__ sigma, double* __restrict__ out) {
+// Sum of nparts block partials of packed Grams, per entry (grid over
+// nmodes x T entries; each thread adds its entry's partials in order).
+__global__ void k_gram_reduce(const double* __restrict__ parts, uint32_t nparts, uint32_t T, uint32_t nmodes,
+                              double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const uint64_t n = (uint64_t)nmodes * T;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = w / T, e = w - k * T;
+        const double* P = parts + k * nparts * T;
+        double s = 0.0;
+        for (uint32_t p = 0; p < nparts; p++) s += P[(uint64_t)p * T + e];
+        out[w] = s;
+    }
+}
+
+// ||M||^2 = sum_{a,b} sigma_a sigma_b prod_k G_k[a][b] (common.h:152-226) from
+// the reduced packed Grams G (nmodes x T); one block of 256 threads.
+__global__ void __launch_bounds__(256) k_chain_final(const double* __restrict__ G, uint32_t N, uint32_t R,
+                                                     const double* __restrict__ sigma, double* __restrict__ out) {
 #pragma clang fp contract(off)
     __shared__ double red[256];
     const uint32_t T = R * (R + 1) / 2;
@@ -655,21 +748,16 @@ __global__ void k_chain_final(const double* __restrict__ gram_parts, uint32_t np
         }
         const uint32_t b = a + (e - base);
         double v = sigma[a] * sigma[b];
-        for (uint32_t k = 0; k < N; k++) {
-            const double* P = gram_parts + (uint64_t)k * nparts * T;
-            double g = 0.0;
-            for (uint32_t p = 0; p < nparts; p++) g += P[(uint64_t)p * T + e];
-            v = v * g;
-        }
+        for (uint32_t k = 0; k < N; k++) v = v * G[(uint64_t)k * T + e];
         acc += a == b ? v : 2.0 * v;
     }
     red[threadIdx.x] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (uint32_t t = 0; t < blockDim.x; t++) s += red[t];
-        out[0] = s;
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 }  // namespace ftl

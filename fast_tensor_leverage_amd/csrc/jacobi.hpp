@@ -195,6 +195,12 @@ __device__ void jacobi_block(double* A, double* V, int R, double* lam, double* V
 constexpr int kJacobiLdsDoubles = 3 * 65 * 64;
 constexpr int kJacobiThreads = 1024;
 
+// Block size of the one-block Jacobi kernels.  Measured on computeM at R=32
+// (N=3, I=2^23): 1024 threads 80 + 72 us, 256 threads 98 + 127 us -- the
+// element-wise phases around the rotations (P_k, V, E, eigen tree) gain
+// more from the extra waves than the barriers cost.
+inline uint32_t jacobi_threads_for(uint64_t R) { (void)R; return 1024u; }
+
 struct JacobiStore {
     double *A, *V, *vecs;
 };

@@ -67,7 +67,7 @@ struct Sampler {
     uint64_t* samples_dev = nullptr;
     double *h_dev = nullptr, *w_dev = nullptr, *draws_dev = nullptr;
     uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
-    uint32_t sg_override = 0, grid_override = 0;
+    uint32_t sg_override = 0, grid_override = 0, jacobi_override = 0;
     bool timing = false;
     // events of every timed call since the last read-out: {start, after
     // gram_pinv, after eigen_modes, after draw}
@@ -229,6 +229,10 @@ struct Sampler {
         } else if (key == "grid_blocks") {
             FTL_REQUIRE(value >= 0, "grid_blocks must be >= 0");
             grid_override = (uint32_t)value;
+        } else if (key == "jacobi_threads") {
+            FTL_REQUIRE(value == 0 || value == 256 || value == 512 || value == 1024,
+                        "jacobi_threads must be 0, 256, 512 or 1024");
+            jacobi_override = (uint32_t)value;
         } else if (key == "fp32_levels") {
             FTL_REQUIRE(value >= 0 && value <= 64, "fp32_levels must be in [0, 64]");
             fp32_levels = value == 0 ? kDefaultFp32Levels : (uint32_t)value;
@@ -242,6 +246,7 @@ struct Sampler {
     int64_t get_option(const std::string& key) const {
         if (key == "group_lanes") return sg_override;
         if (key == "grid_blocks") return grid_override;
+        if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
         if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
         throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
@@ -279,7 +284,9 @@ struct Sampler {
                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             // computeM blocks hold A ((R+1) R doubles) in LDS, plus V and the
             // eigenvectors when R <= 64: at most 132 KB of dynamic LDS
-            const void* cm[] = {(const void*)k_gram_pinv<kJacobiThreads>, (const void*)k_eigen_modes<kJacobiThreads>};
+            const void* cm[] = {(const void*)k_gram_pinv<256>, (const void*)k_eigen_modes<256>,
+                                (const void*)k_gram_pinv<512>, (const void*)k_eigen_modes<512>,
+                                (const void*)k_gram_pinv<1024>, (const void*)k_eigen_modes<1024>};
             for (const void* f : cm)
                 FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
             attrs_set = true;
@@ -402,7 +409,14 @@ struct Sampler {
         ComputeMArgs a = computeM_args(order);
         const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
         const size_t lds = jac * sizeof(double);
-        constexpr int BS = kJacobiThreads;
+        const uint32_t bs = jacobi_override ? jacobi_override : jacobi_threads_for(R);
+        if (bs == 256) launch_cm<256>(a, lds, st, evs);
+        else if (bs == 512) launch_cm<512>(a, lds, st, evs);
+        else launch_cm<1024>(a, lds, st, evs);
+    }
+
+    template <int BS>
+    void launch_cm(const ComputeMArgs& a, size_t lds, hipStream_t st, hipEvent_t* evs) {
         if (evs) FTL_HIP_CHECK(hipEventRecord(evs[0], st));
         hipLaunchKernelGGL(k_gram_pinv<BS>, dim3(1), dim3(BS), lds, st, a);
         FTL_HIP_CHECK(hipGetLastError());

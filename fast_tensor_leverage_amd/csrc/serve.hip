@@ -83,16 +83,22 @@ private:
     std::map<size_t, size_t> free_;
 };
 
-// Factor entries N(0,1) rounded to FP32 (stored FP64): Box-Muller on two
-// uniforms from one Philox4x32-10 block keyed by (seed, mode, entry).
+// Factor entries N(0,1) in FP32 (stored FP64): Box-Muller in FP32 on two
+// 24-bit uniforms of one Philox4x32-10 block keyed by (seed, mode, entry
+// pair); each block gives two entries.
 __global__ void k_normal_factor(double* __restrict__ U, uint64_t n, uint64_t seed, uint32_t mode) {
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t c[4] = {(uint32_t)e, (uint32_t)(e >> 32), mode, 0x9E3779B9u};
+    const uint64_t pairs = (n + 1) / 2;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), mode, 0x9E3779B9u};
         philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-        const double u1 = ((double)((uint64_t)(c[0] >> 5) * 67108864ull + (c[1] >> 6)) + 1.0) * (1.0 / 9007199254740993.0);
-        const double u2 = (double)((uint64_t)(c[2] >> 5) * 67108864ull + (c[3] >> 6)) * (1.0 / 9007199254740992.0);
-        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-        U[e] = (double)(float)z;
+        const float u1 = ((float)(c[0] >> 8) + 1.0f) * (1.0f / 16777217.0f);  // (0, 1)
+        const float u2 = (float)(c[1] >> 8) * (1.0f / 16777216.0f);           // [0, 1)
+        const float rad = sqrtf(-2.0f * logf(u1));
+        float sn, cs;
+        sincosf(6.2831853f * u2, &sn, &cs);
+        U[2 * p] = (double)(rad * cs);
+        if (2 * p + 1 < n) U[2 * p + 1] = (double)(rad * sn);
     }
 }
 
@@ -215,7 +221,7 @@ struct Server {
             r->samples = (uint64_t*)take(r->N * r->J * sizeof(uint64_t));
             for (uint32_t k = 0; k < r->N; k++) {
                 const uint64_t n = r->I[k] * r->R;
-                const unsigned g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (n + 255) / 256), 16384);
+                const unsigned g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (n / 2 + 255) / 256), 16384);
                 hipLaunchKernelGGL(k_normal_factor, dim3(g), dim3(256), 0, st, r->U[k], n, r->seed, k);
                 FTL_HIP_CHECK(hipGetLastError());
             }
