@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ftl_common.hpp"
@@ -68,6 +69,20 @@ struct Sampler {
     double *h_dev = nullptr, *w_dev = nullptr, *draws_dev = nullptr;
     uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
     uint32_t sg_override = 0, grid_override = 0, jacobi_override = 0;
+    // 1: one draw launch per sampled mode (h kept in h_state between them),
+    // so a launch's working set is one tree; 0: all modes in one launch
+    uint32_t mode_split = 1;
+    // draw kernel: 0 = a group of lanes per sample, 1 = one sample per wave
+    uint32_t draw_kernel = 0;
+    // 1: each mode's descent bucketed by level-3 subtree onto XCD-affine
+    // blocks.  Measured at N=3, I=2^23, R=32 (FP64): prefix 167 + partition
+    // 97 + suffix 367 us per mode against 539 us for the one-launch mode, so
+    // off by default: the per-level cost does not depend on where the node
+    // is cached.
+    uint32_t xcd_buckets = 0;
+    double *st_x = nullptr, *st_s = nullptr;
+    uint32_t *bucket = nullptr, *blist = nullptr, *boff = nullptr;
+    double* h_state = nullptr;
     bool timing = false;
     // events of every timed call since the last read-out: {start, after
     // gram_pinv, after eigen_modes, after draw}
@@ -229,6 +244,15 @@ struct Sampler {
         } else if (key == "grid_blocks") {
             FTL_REQUIRE(value >= 0, "grid_blocks must be >= 0");
             grid_override = (uint32_t)value;
+        } else if (key == "xcd_buckets") {
+            FTL_REQUIRE(value == 0 || value == 1, "xcd_buckets must be 0 or 1");
+            xcd_buckets = (uint32_t)value;
+        } else if (key == "draw_kernel") {
+            FTL_REQUIRE(value == 0 || value == 1, "draw_kernel must be 0 or 1");
+            draw_kernel = (uint32_t)value;
+        } else if (key == "mode_split") {
+            FTL_REQUIRE(value == 0 || value == 1, "mode_split must be 0 or 1");
+            mode_split = (uint32_t)value;
         } else if (key == "jacobi_threads") {
             FTL_REQUIRE(value == 0 || value == 256 || value == 512 || value == 1024,
                         "jacobi_threads must be 0, 256, 512 or 1024");
@@ -246,6 +270,9 @@ struct Sampler {
     int64_t get_option(const std::string& key) const {
         if (key == "group_lanes") return sg_override;
         if (key == "grid_blocks") return grid_override;
+        if (key == "mode_split") return mode_split;
+        if (key == "draw_kernel") return draw_kernel;
+        if (key == "xcd_buckets") return xcd_buckets;
         if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
         if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
@@ -334,6 +361,12 @@ struct Sampler {
         etree = dalloc<double>((uint64_t)N * etree_nodes * ts64 + pad);
         work = dalloc<double>((uint64_t)(N + 1) * 3 * (R + 1) * R);
         lam = dalloc<double>((uint64_t)(N + 1) * R);
+        h_state = dalloc<double>(J * R);
+        st_x = dalloc<double>(J * R);
+        st_s = dalloc<double>(J * 4);
+        bucket = dalloc<uint32_t>(J);
+        blist = dalloc<uint32_t>(J);
+        boff = dalloc<uint32_t>(kBuckets + 1);
         FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R2 * sizeof(double), stream));
         FTL_HIP_CHECK(hipMemsetAsync(etree, 0, ((uint64_t)N * etree_nodes * ts64 + pad) * sizeof(double), stream));
         if (sync_end) FTL_HIP_CHECK(hipStreamSynchronize(stream));
@@ -341,7 +374,8 @@ struct Sampler {
 
     // Device bytes init(alias = true) allocates, each block rounded up to
     // `align` (the factors are the caller's).
-    static size_t footprint(uint32_t N, const uint64_t* I, uint64_t R, int prec, uint32_t fp32_lv, size_t align) {
+    static size_t footprint(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int prec, uint32_t fp32_lv,
+                            size_t align) {
         auto up = [&](size_t b) { return (std::max<size_t>(b, 16) + align - 1) / align * align; };
         const uint64_t ts64 = tri_stride64(R), ts32 = tri_stride32(R);
         uint64_t rmax = 8;
@@ -367,6 +401,7 @@ struct Sampler {
         const uint64_t R2 = R * R;
         total += up(R2 * 8) * 2 + up(N * R2 * 8) + up(((uint64_t)N * (2 * R - 1) * ts64 + pad) * 8) +
                  up((uint64_t)(N + 1) * 3 * (R + 1) * R * 8) + up((uint64_t)(N + 1) * R * 8);
+        total += up(J * R * 8) * 2 + up(J * 4 * 8) + up(J * 4) * 2 + up((kBuckets + 1) * 4);
         return total;
     }
 
@@ -445,26 +480,80 @@ struct Sampler {
             launch_draw<RMAX, SG, false>(a, st);
     }
 
-    void dispatch_draw(const DrawArgs& a, hipStream_t st) {
+    template <int RMAX, bool USE32>
+    void launch_draw_wave(const DrawArgs& a, hipStream_t st) {
+        constexpr int BS = 256;
+        uint64_t blocks = (J + 3) / 4;
+        if (grid_override) blocks = std::min<uint64_t>(blocks, grid_override);
+        blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 20));
+        hipLaunchKernelGGL((k_krp_draw_wave<RMAX, BS, USE32>), dim3((unsigned)blocks), dim3(BS), 0, st, a);
+        FTL_HIP_CHECK(hipGetLastError());
+    }
+
+    // Calls f(RMAX, SG) (as integral constants) for the draw configuration of
+    // this sampler.  Measured at N=3, I=2^23, R=32, J=5e4: SG=32 fastest for
+    // both precisions (FP64 1.56 ms vs 1.64 at SG=16; FP32 1.15 vs 1.33).
+    template <typename F>
+    void with_cfg(F&& f) const {
+        using std::integral_constant;
         const uint32_t sg = sg_override;
         if (R <= 8) {
-            launch_draw_p<8, 8>(a, st);
+            f(integral_constant<int, 8>{}, integral_constant<int, 8>{});
         } else if (R <= 16) {
-            launch_draw_p<16, 16>(a, st);
+            f(integral_constant<int, 16>{}, integral_constant<int, 16>{});
         } else if (R <= 32) {
-            // measured at N=3, I=2^23, R=32, J=5e4: SG=32 fastest for both
-            // precisions (FP64 1.56 ms vs 1.64 at SG=16; FP32 1.15 vs 1.33)
             const uint32_t s = sg ? sg : 32u;
-            if (s == 8) launch_draw_p<32, 8>(a, st);
-            else if (s == 16) launch_draw_p<32, 16>(a, st);
-            else if (s == 64) launch_draw_p<32, 64>(a, st);
-            else launch_draw_p<32, 32>(a, st);
+            if (s == 8) f(integral_constant<int, 32>{}, integral_constant<int, 8>{});
+            else if (s == 16) f(integral_constant<int, 32>{}, integral_constant<int, 16>{});
+            else if (s == 64) f(integral_constant<int, 32>{}, integral_constant<int, 64>{});
+            else f(integral_constant<int, 32>{}, integral_constant<int, 32>{});
         } else if (R <= 64) {
-            if (sg == 32) launch_draw_p<64, 32>(a, st);
-            else launch_draw_p<64, 64>(a, st);
+            if (sg == 32) f(integral_constant<int, 64>{}, integral_constant<int, 32>{});
+            else f(integral_constant<int, 64>{}, integral_constant<int, 64>{});
         } else {
-            launch_draw_p<128, 64>(a, st);
+            f(integral_constant<int, 128>{}, integral_constant<int, 64>{});
         }
+    }
+
+    void dispatch_draw(const DrawArgs& a, hipStream_t st) {
+        const uint32_t sg = sg_override;
+        if (draw_kernel == 1 && R <= 32 && sg == 0) {
+            if (R <= 8) {
+                if (precision == FTL_FP32) launch_draw_wave<8, true>(a, st);
+                else launch_draw_wave<8, false>(a, st);
+            } else if (R <= 16) {
+                if (precision == FTL_FP32) launch_draw_wave<16, true>(a, st);
+                else launch_draw_wave<16, false>(a, st);
+            } else {
+                if (precision == FTL_FP32) launch_draw_wave<32, true>(a, st);
+                else launch_draw_wave<32, false>(a, st);
+            }
+            return;
+        }
+        with_cfg([&](auto rmax, auto sgc) { launch_draw_p<decltype(rmax)::value, decltype(sgc)::value>(a, st); });
+    }
+
+    // The bucketed draw of sampled mode a.mi_begin: prefix, partition, suffix.
+    void dispatch_bucketed(const DrawArgs& a, hipStream_t st) {
+        with_cfg([&](auto rmax, auto sgc) {
+            constexpr int RM = decltype(rmax)::value, SG = decltype(sgc)::value, BS = 256;
+            const uint64_t per_block = BS / SG;
+            const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((J + per_block - 1) / per_block, 1u << 20));
+            hipLaunchKernelGGL((k_draw_prefix<RM, SG, BS>), dim3((unsigned)blocks), dim3(BS), 0, st, a);
+            FTL_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_bucket_partition<1024>, dim3(1), dim3(1024), 0, st, a.bucket, J, a.blist, a.boff);
+            FTL_HIP_CHECK(hipGetLastError());
+            // kBuckets x the blocks one bucket of J / kBuckets samples needs
+            const uint64_t per_bucket = (J + kBuckets - 1) / kBuckets;
+            const uint64_t sblocks = kBuckets * std::max<uint64_t>(1, (per_bucket + per_block - 1) / per_block);
+            if (precision == FTL_FP32)
+                hipLaunchKernelGGL((k_draw_suffix<RM, SG, BS, true>), dim3((unsigned)std::min<uint64_t>(sblocks, 1u << 20)),
+                                   dim3(BS), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_draw_suffix<RM, SG, BS, false>), dim3((unsigned)std::min<uint64_t>(sblocks, 1u << 20)),
+                                   dim3(BS), 0, st, a);
+            FTL_HIP_CHECK(hipGetLastError());
+        });
     }
 
     void draw_device(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws, hipStream_t st) {
@@ -504,7 +593,27 @@ struct Sampler {
         a.samples = samples;
         a.h_out = h;
         a.w_out = w;
-        dispatch_draw(a, st);
+        a.split_level = kBucketLevel;
+        a.st_x = st_x;
+        a.st_s = st_s;
+        a.bucket = bucket;
+        a.blist = blist;
+        a.boff = boff;
+        if (mode_split && h_state) {
+            a.h_state = h_state;
+            for (int mi = 0; mi < a.m; mi++) {
+                a.mi_begin = mi;
+                a.mi_end = mi + 1;
+                // bucketed when the tree is deep enough for the split to matter
+                if (xcd_buckets && draw_kernel == 0 && modes[a.order[mi]].sh.lfill_level >= 8) dispatch_bucketed(a, st);
+                else dispatch_draw(a, st);
+            }
+        } else {
+            a.mi_begin = 0;
+            a.mi_end = a.m;
+            a.h_state = nullptr;
+            dispatch_draw(a, st);
+        }
         if (timing) {
             FTL_HIP_CHECK(hipEventRecord(evs[3], st));
             pending.push_back(evs);
@@ -583,8 +692,8 @@ Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R
 
 hipStream_t sampler_stream(Sampler* s) { return s->stream; }
 
-size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t R, int precision, size_t align) {
-    return Sampler::footprint(N, I, R, precision, kDefaultFp32Levels, align);
+size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int precision, size_t align) {
+    return Sampler::footprint(N, I, J, R, precision, kDefaultFp32Levels, align);
 }
 
 Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,

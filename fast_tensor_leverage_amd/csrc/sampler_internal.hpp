@@ -30,7 +30,7 @@ Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R
                            DeviceAllocator* alloc, hipStream_t st);
 hipStream_t sampler_stream(Sampler* s);
 // Device bytes sampler_new_async allocates (blocks rounded up to `align`).
-size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t R, int precision, size_t align);
+size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int precision, size_t align);
 // KRPDrawSamples(j) into device buffers (samples N x J, h J x R, w J); draws
 // NULL -> the built-in Philox uniforms of the next call
 void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* w, const double* draws,

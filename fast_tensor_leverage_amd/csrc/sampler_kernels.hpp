@@ -282,14 +282,39 @@ struct DrawArgs {
     uint64_t* samples;     // N x J
     double* h_out;         // J x R or null
     double* w_out;         // J or null
+    // sampled-mode positions [mi_begin, mi_end) of this launch; h carried
+    // between launches in h_state (J x R) when the draw is split by mode
+    int mi_begin, mi_end;
+    double* h_state;
+    // subtree-bucketed descent of one mode (k_draw_prefix / k_draw_suffix):
+    // per-sample x (J x R) and {low, high, m, node} (J x 4) after the first
+    // split_level levels, the level-split_level subtree of each sample and
+    // the samples grouped by subtree (offsets: kBuckets + 1 entries)
+    uint32_t split_level;
+    double* st_x;
+    double* st_s;
+    uint32_t* bucket;
+    uint32_t* blist;
+    uint32_t* boff;
 };
+
+// Subtrees of the bucketed descent: one per XCD (8 on MI355X).  Workgroups
+// are dispatched to the XCDs round-robin, so block b runs on XCD b % 8 and
+// the suffix kernel gives bucket b % 8 to block b: each XCD's L2 then holds
+// the nodes of one subtree only, eight times fewer than the whole tree.
+constexpr uint32_t kBuckets = 8;
+constexpr uint32_t kBucketLevel = 3;
 
 template <int RMAX, int SG>
 struct DrawCfg {
     static constexpr int TMAX = RMAX * (RMAX + 1) / 2;
     static constexpr int EPL = (TMAX + SG - 1) / SG;   // packed entries per lane
     static constexpr int HPL = (RMAX + SG - 1) / SG;   // vector entries per lane
-    static constexpr int RPL = (RMAX + SG - 1) / SG;   // leaf rows per lane
+    static constexpr int RPL = (RMAX + SG - 1) / SG;   // leaf rows per lane (scan)
+    // leaf dot products: LPR lanes per row, 8 columns per lane
+    static constexpr int LCOLS = 8;
+    static constexpr int LPR = RMAX / LCOLS;
+    static constexpr int RPP = SG / LPR;                // rows per pass
 };
 
 // Coefficients of x^T G x over the packed entries this lane owns:
@@ -391,62 +416,103 @@ __device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t 
     return c;
 }
 
-// U_row . x for one leaf row, accumulated in FP64.  With R == RMAX the row is
-// read with 16-byte vector loads (rows are 16-byte aligned then); otherwise
-// entry by entry with the column index clamped (x is zero past R).  All loads
-// are unconditional.
-template <typename TU, int RMAX>
-__device__ inline double row_dot(const TU* __restrict__ row, const double* xs, int R) {
-    constexpr int W = 16 / sizeof(TU);
-    typedef TU V __attribute__((ext_vector_type(W)));
-    double acc0 = 0, acc1 = 0;
-    if (R == RMAX) {
-        const V* rv = reinterpret_cast<const V*>(row);
-        V buf[RMAX / W];
-#pragma unroll
-        for (int q = 0; q < RMAX / W; q++) buf[q] = rv[q];
-#pragma unroll
-        for (int q = 0; q < RMAX / W; q++)
-#pragma unroll
-            for (int w = 0; w < W; w++) {
-                if (w & 1) acc1 += (double)buf[q][w] * xs[q * W + w];
-                else acc0 += (double)buf[q][w] * xs[q * W + w];
-            }
-    } else {
-        TU buf[RMAX];
-#pragma unroll
-        for (int b = 0; b < RMAX; b++) buf[b] = row[b < R ? b : R - 1];
-#pragma unroll
-        for (int b = 0; b < RMAX; b++) {
-            if (b & 1) acc1 += (double)buf[b] * xs[b];
-            else acc0 += (double)buf[b] * xs[b];
+// Descent steps d_begin .. d_end-1 (and the guarded tail level when tail)
+// from the state (c, low, high) with root mass m: the loop of descend, split
+// so that a descent can stop after the top levels and resume elsewhere.
+template <int RMAX, int SG, bool USE32>
+__device__ inline void descend_steps(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
+                                     const double (&c64)[DrawCfg<RMAX, SG>::EPL], const double* tree64, double u,
+                                     int g, double m, uint32_t d_begin, uint32_t d_end, bool tail, uint64_t& c,
+                                     double& low, double& high) {
+    constexpr int EPL = DrawCfg<RMAX, SG>::EPL;
+    for (uint32_t d = d_begin; d < d_end; d++) {
+        const uint64_t child = 2 * c + 1;
+        const double mL = child_qform<EPL, SG, USE32>(md, tree64, child, d + 1, ts64, ts32, c64, g);
+        const double cutoff = low + mL / m;
+        if (u <= cutoff) {
+            c = child;
+            high = cutoff;
+        } else {
+            c = child + 1;
+            low = cutoff;
         }
     }
-    return acc0 + acc1;
+    if (tail && sh.node_count > sh.nodes_before_lfill) {
+        const bool leaf = 2 * c + 1 >= sh.node_count;
+        if (!leaf) {
+            const uint64_t child = 2 * c + 1;
+            const double mL =
+                child_qform<EPL, SG, USE32>(md, tree64, child, (uint32_t)sh.lfill_level + 1, ts64, ts32, c64, g);
+            const double cutoff = low + mL / m;
+            if (u <= cutoff) {
+                c = child;
+                high = cutoff;
+            } else {
+                c = child + 1;
+                low = cutoff;
+            }
+        }
+    }
 }
 
 // Leaf row scan (partition_tree.hpp:336-391): q_t = (U_t . x)^2 for the rows
 // of the leaf, inclusive prefix over the group, P_{t+1} = prefix_t / total,
 // result = first t <= F-2 with frac < P_{t+1}, else F-1 (clamped to the last
-// real row of a partial leaf).  Lane g owns rows g, g+SG, ...
+// real row of a partial leaf).
+// The dot products take LPR lanes per row, 8 consecutive columns per lane
+// (16-byte loads when R == RMAX, else element loads with the column clamped,
+// x being zero past R), summed over the LPR lanes with the DPP butterfly, and
+// RPP = SG / LPR rows per pass; q goes to LDS (qs, RMAX entries per group)
+// and lane g then owns rows g, g+SG, ... for the scan.  Few registers per
+// lane: the kernel's occupancy, not its bandwidth, bounds the descents.
 template <typename TU, int RMAX, int SG>
 __device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint32_t rows, uint32_t F,
-                                     int R, const double* xs, double frac, int g, int lane) {
-    constexpr int RPL = DrawCfg<RMAX, SG>::RPL;
-    double q2[RPL];
-#pragma unroll
-    for (int k = 0; k < RPL; k++) {
-        const uint32_t t = g + SG * k;
+                                     int R, const double* xs, double* qs, double frac, int g, int lane) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    constexpr int LC = Cfg::LCOLS, LPR = Cfg::LPR, RPP = Cfg::RPP, RPL = Cfg::RPL;
+    constexpr int W = 16 / sizeof(TU);
+    typedef TU V __attribute__((ext_vector_type(W)));
+    const int sub = g % LPR, rsub = g / LPR;
+    const int col0 = sub * LC;
+    const uint32_t passes = (F + RPP - 1) / RPP;
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t t = p * RPP + rsub;
         const uint32_t tc = t < rows ? t : rows - 1;
-        const double d = row_dot<TU, RMAX>(U + (r0 + tc) * (uint64_t)R, xs, R);
-        q2[k] = t < rows ? d * d : 0.0;
+        const TU* row = U + (r0 + tc) * (uint64_t)R;
+        TU buf[LC];
+        if (R == RMAX) {
+            const V* rv = reinterpret_cast<const V*>(row + col0);
+#pragma unroll
+            for (int q = 0; q < LC / W; q++) {
+                const V v = rv[q];
+#pragma unroll
+                for (int w = 0; w < W; w++) buf[q * W + w] = v[w];
+            }
+        } else {
+#pragma unroll
+            for (int w = 0; w < LC; w++) {
+                const int b = col0 + w;
+                buf[w] = row[b < R ? b : R - 1];
+            }
+        }
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int w = 0; w < LC; w += 2) {
+            acc0 += (double)buf[w] * xs[col0 + w];
+            acc1 += (double)buf[w + 1] * xs[col0 + w + 1];
+        }
+        const double d = group_sum<LPR>(acc0 + acc1);
+        if (sub == 0 && t < F) qs[t] = t < rows ? d * d : 0.0;
     }
+    wave_sync_lds();
     double incl[RPL];
     double run = 0.0;
 #pragma unroll
     for (int k = 0; k < RPL; k++) {
         if (SG * k < (int)F) {
-            const double v = group_incl_scan<SG>(q2[k], g) + run;
+            const uint32_t t = g + SG * k;
+            const double q2 = t < F ? qs[t] : 0.0;
+            const double v = group_incl_scan<SG>(q2, g) + run;
             incl[k] = v;
             run = __shfl(v, SG - 1, SG);
         } else {
@@ -479,11 +545,13 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
     constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
     constexpr int GPW = 64 / SG;  // groups per wave
     __shared__ double lds_x[BS / SG][RMAX];
+    __shared__ double lds_q[BS / SG][RMAX];
     const int lane = threadIdx.x & 63;
     const int g = lane % SG;
     const int wave = threadIdx.x / 64;
     const int grp_in_block = threadIdx.x / SG;
     double* xs = lds_x[grp_in_block];
+    double* qs = lds_q[grp_in_block];
     const int R = a.R;
     const int T = (int)tri_size(R);
     const int epl = (T + SG - 1) / SG;
@@ -503,8 +571,13 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
         double h[HPL];
 #pragma unroll
         for (int k = 0; k < HPL; k++) h[k] = 1.0;
+        if (a.mi_begin > 0 && active) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) h[q] = a.h_state[i * R + g + SG * q];
+        }
 
-        for (int mi = 0; mi < a.m; mi++) {
+        for (int mi = a.mi_begin; mi < a.mi_end; mi++) {
             const int k = a.order[mi];
             const ModeDev& md = a.mode[k];
             // ---- eigen stage: x = h -----------------------------------------
@@ -540,9 +613,9 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 const uint64_t r0 = idx * md.sh.F;
                 const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
                 if constexpr (fp32)
-                    res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, frac, g, lane);
+                    res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
                 else
-                    res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, frac, g,
+                    res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g,
                                                       lane);
             }
             const uint64_t srow = idx * md.sh.F + res;
@@ -551,6 +624,401 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 if (g + SG * q < R) h[q] *= md.U64[srow * R + g + SG * q];
             if (active && g == 0) a.samples[(uint64_t)k * a.J + i] = srow;
             wave_sync_lds();
+        }
+        if (a.mi_end < a.m) {  // the next launch continues from h
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < HPL; q++)
+                    if (g + SG * q < R) a.h_state[i * R + g + SG * q] = h[q];
+            }
+            continue;
+        }
+        // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = h[q];
+        wave_sync_lds();
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < HPL; q++) {
+            const int ra = g + SG * q;
+            if (ra < R) {
+                double y = 0.0;
+                for (int b = 0; b < R; b++) y += a.pinv[ra * R + b] * xs[b];
+                part += h[q] * y;
+            }
+        }
+        const double lev = group_sum<SG>(part);
+        if (active) {
+            if (a.w_out && g == 0) a.w_out[i] = (double)R / (lev * (double)a.J);
+            if (a.h_out) {
+#pragma unroll
+                for (int q = 0; q < HPL; q++)
+                    if (g + SG * q < R) a.h_out[i * R + g + SG * q] = h[q];
+            }
+        }
+        wave_sync_lds();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One-sample-per-wave draw kernel.  The whole wave (64 lanes) works on one
+// sample, so every per-sample quantity (cutoffs, low/high, node index, the
+// uniform) is wave-uniform and lives in scalar registers, the descent branch
+// is a scalar branch, and each lane holds only ceil(T/64) coefficients and
+// node values.  The lower register count admits more waves per SIMD, i.e.
+// more descents in flight: the descents are chains of dependent node reads,
+// bounded by how many are in flight rather than by bandwidth.  Same
+// arithmetic and comparisons as k_krp_draw.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int RMAX, bool USE32>
+__device__ inline uint64_t descend_wave(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
+                                        const double (&c64)[DrawCfg<RMAX, 64>::EPL], const double* tree64, double u,
+                                        int g, double& low, double& high) {
+    constexpr int EPL = DrawCfg<RMAX, 64>::EPL;
+    const double m = wave_uniform(node_qform<double, EPL, 64>(tree64, c64, g));
+    uint64_t c = 0;
+    double lo = 0.0, hi = 1.0;
+    for (uint32_t d = 0; d < sh.lfill_level; d++) {
+        const uint64_t child = 2 * c + 1;
+        const double mL = wave_uniform(child_qform<EPL, 64, USE32>(md, tree64, child, d + 1, ts64, ts32, c64, g));
+        const double cutoff = lo + mL / m;
+        if (u <= cutoff) {
+            c = child;
+            hi = cutoff;
+        } else {
+            c = child + 1;
+            lo = cutoff;
+        }
+    }
+    if (sh.node_count > sh.nodes_before_lfill) {
+        const bool leaf = 2 * c + 1 >= sh.node_count;
+        if (!leaf) {
+            const uint64_t child = 2 * c + 1;
+            const double mL = wave_uniform(
+                child_qform<EPL, 64, USE32>(md, tree64, child, (uint32_t)sh.lfill_level + 1, ts64, ts32, c64, g));
+            const double cutoff = lo + mL / m;
+            if (u <= cutoff) {
+                c = child;
+                hi = cutoff;
+            } else {
+                c = child + 1;
+                lo = cutoff;
+            }
+        }
+    }
+    low = lo;
+    high = hi;
+    return c;
+}
+
+template <int RMAX, int BS, bool USE32>
+__global__ void __launch_bounds__(BS) k_krp_draw_wave(DrawArgs a) {
+    using Cfg = DrawCfg<RMAX, 64>;
+    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
+    constexpr int WPB = BS / 64;
+    __shared__ double lds_x[WPB][RMAX];
+    __shared__ double lds_q[WPB][RMAX];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double* xs = lds_x[wave];
+    double* qs = lds_q[wave];
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    const int epl = (T + 63) / 64;
+    for (int b = lane; b < RMAX; b += 64) xs[b] = 0.0;
+    wave_sync_lds();
+    const uint64_t waves_total = (uint64_t)gridDim.x * WPB;
+    for (uint64_t i = (uint64_t)blockIdx.x * WPB + wave; i < a.J; i += waves_total) {
+        double h[HPL];
+#pragma unroll
+        for (int q = 0; q < HPL; q++) h[q] = 1.0;
+        if (a.mi_begin > 0) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) h[q] = a.h_state[i * R + lane + 64 * q];
+        }
+        for (int mi = a.mi_begin; mi < a.mi_end; mi++) {
+            const int k = a.order[mi];
+            const ModeDev& md = a.mode[k];
+            // ---- eigen stage: x = h ---------------------------------------
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) xs[lane + 64 * q] = h[q];
+            wave_sync_lds();
+            double c64[EPL];
+            make_coefs<RMAX, 64>(xs, R, lane, epl, c64);
+            const double ue = wave_uniform(a.draws ? a.draws[(uint64_t)k * a.J + i]
+                                                   : philox_uniform(a.seed, a.call, i, 2 * k));
+            double low, high;
+            const uint64_t ce = descend_wave<RMAX, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, lane, low, high);
+            const uint64_t r = leaf_index(a.esh, ce);
+            wave_sync_lds();
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) xs[lane + 64 * q] = h[q] * md.E[r * R + lane + 64 * q];
+            wave_sync_lds();
+            // ---- Gram stage: x = scaled_h -----------------------------------
+            make_coefs<RMAX, 64>(xs, R, lane, epl, c64);
+            const double ug = wave_uniform(a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                                   : philox_uniform(a.seed, a.call, i, 2 * k + 1));
+            const uint64_t c = descend_wave<RMAX, USE32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, lane, low, high);
+            const uint64_t idx = leaf_index(md.sh, c);
+            uint32_t res = 0;
+            if (md.sh.F > 1) {
+                const double frac = (ug - low) / (high - low);
+                const uint64_t r0 = idx * md.sh.F;
+                const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
+                if constexpr (USE32)
+                    res = leaf_scan<float, RMAX, 64>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, lane, lane);
+                else
+                    res = leaf_scan<double, RMAX, 64>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, lane,
+                                                      lane);
+            }
+            const uint64_t srow = idx * md.sh.F + res;
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) h[q] *= md.U64[srow * R + lane + 64 * q];
+            if (lane == 0) a.samples[(uint64_t)k * a.J + i] = srow;
+            wave_sync_lds();
+        }
+        if (a.mi_end < a.m) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) a.h_state[i * R + lane + 64 * q] = h[q];
+            continue;
+        }
+        // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (lane + 64 * q < R) xs[lane + 64 * q] = h[q];
+        wave_sync_lds();
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < HPL; q++) {
+            const int ra = lane + 64 * q;
+            if (ra < R) {
+                double y = 0.0;
+                for (int b = 0; b < R; b++) y += a.pinv[ra * R + b] * xs[b];
+                part += h[q] * y;
+            }
+        }
+        const double lev = group_sum<64>(part);
+        if (a.w_out && lane == 0) a.w_out[i] = (double)R / (lev * (double)a.J);
+        if (a.h_out) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (lane + 64 * q < R) a.h_out[i * R + lane + 64 * q] = h[q];
+        }
+        wave_sync_lds();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Subtree-bucketed draw of one sampled mode (positions mi_begin, one launch
+// each): k_draw_prefix runs the eigen stage and the first split_level Gram
+// levels and records the state, k_bucket_partition groups the samples by
+// their subtree, k_draw_suffix finishes the descents bucket by bucket on
+// XCD-affine blocks.  The arithmetic and comparisons are those of k_krp_draw,
+// so the results are identical.
+// ---------------------------------------------------------------------------
+template <int RMAX, int SG, int BS>
+__global__ void __launch_bounds__(BS) k_draw_prefix(DrawArgs a) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
+    constexpr int GPW = 64 / SG;
+    __shared__ double lds_x[BS / SG][RMAX];
+    const int lane = threadIdx.x & 63;
+    const int g = lane % SG;
+    const int wave = threadIdx.x / 64;
+    double* xs = lds_x[threadIdx.x / SG];
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    const int epl = (T + SG - 1) / SG;
+    for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
+    wave_sync_lds();
+    const int mi = a.mi_begin;
+    const int k = a.order[mi];
+    const ModeDev& md = a.mode[k];
+    const uint64_t waves_total = (uint64_t)gridDim.x * (BS / 64);
+    const uint64_t n_items = (a.J + GPW - 1) / GPW;
+    for (uint64_t item = (uint64_t)blockIdx.x * (BS / 64) + wave; item < n_items; item += waves_total) {
+        const uint64_t i = item * GPW + lane / SG;
+        const bool active = i < a.J;
+        double h[HPL];
+#pragma unroll
+        for (int q = 0; q < HPL; q++) h[q] = 1.0;
+        if (mi > 0 && active) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) h[q] = a.h_state[i * R + g + SG * q];
+        }
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = h[q];
+        wave_sync_lds();
+        double c64[EPL];
+        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        const double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k))
+                                 : 0.5;
+        double low, high;
+        const uint64_t ce = descend<RMAX, SG, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, g, low, high);
+        const uint64_t r = leaf_index(a.esh, ce);
+        wave_sync_lds();
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) {
+                const double x = h[q] * md.E[r * R + g + SG * q];
+                xs[g + SG * q] = x;
+                if (active) a.st_x[i * R + g + SG * q] = x;
+            }
+        wave_sync_lds();
+        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                            : philox_uniform(a.seed, a.call, i, 2 * k + 1))
+                                 : 0.5;
+        const double m = node_qform<double, EPL, SG>(md.tree64, c64, g);
+        uint64_t c = 0;
+        low = 0.0;
+        high = 1.0;
+        descend_steps<RMAX, SG, false>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, g, m, 0, a.split_level, false, c,
+                                       low, high);
+        if (active && g == 0) {
+            a.st_s[i * 4 + 0] = low;
+            a.st_s[i * 4 + 1] = high;
+            a.st_s[i * 4 + 2] = m;
+            a.st_s[i * 4 + 3] = (double)c;
+            a.bucket[i] = (uint32_t)(c - ((1ull << a.split_level) - 1)) % kBuckets;
+        }
+        wave_sync_lds();
+    }
+}
+
+// Stable partition of the samples by bucket (one block): list holds the
+// sample indices bucket by bucket, off[b] the start of bucket b.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_bucket_partition(const uint32_t* __restrict__ bucket, uint64_t J,
+                                                         uint32_t* __restrict__ list, uint32_t* __restrict__ off) {
+    __shared__ uint32_t cnt[kBuckets][BS];
+    __shared__ uint32_t base[kBuckets + 1];
+    const uint32_t t = threadIdx.x;
+    const uint64_t chunk = (J + BS - 1) / BS;
+    const uint64_t s0 = min<uint64_t>(J, t * chunk), s1 = min<uint64_t>(J, s0 + chunk);
+    uint32_t c[kBuckets];
+#pragma unroll
+    for (uint32_t b = 0; b < kBuckets; b++) c[b] = 0;
+    for (uint64_t s = s0; s < s1; s++) c[bucket[s]]++;
+#pragma unroll
+    for (uint32_t b = 0; b < kBuckets; b++) cnt[b][t] = c[b];
+    __syncthreads();
+    if (t < kBuckets) {  // exclusive scan of bucket t over the threads
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < BS; q++) {
+            const uint32_t v = cnt[t][q];
+            cnt[t][q] = run;
+            run += v;
+        }
+        base[t + 1] = run;  // size of bucket t
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < kBuckets; b++) {
+            const uint32_t n = base[b + 1];
+            base[b] = run;
+            run += n;
+            off[b] = base[b];
+        }
+        base[kBuckets] = run;
+        off[kBuckets] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t b = 0; b < kBuckets; b++) c[b] = base[b] + cnt[b][t];
+    for (uint64_t s = s0; s < s1; s++) list[c[bucket[s]]++] = (uint32_t)s;
+}
+
+template <int RMAX, int SG, int BS, bool USE32>
+__global__ void __launch_bounds__(BS) k_draw_suffix(DrawArgs a) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
+    constexpr int GPW = 64 / SG;
+    __shared__ double lds_x[BS / SG][RMAX];
+    __shared__ double lds_q[BS / SG][RMAX];
+    const int lane = threadIdx.x & 63;
+    const int g = lane % SG;
+    const int wave = threadIdx.x / 64;
+    double* xs = lds_x[threadIdx.x / SG];
+    double* qs = lds_q[threadIdx.x / SG];
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    const int epl = (T + SG - 1) / SG;
+    for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
+    wave_sync_lds();
+    const int mi = a.mi_begin;
+    const int k = a.order[mi];
+    const ModeDev& md = a.mode[k];
+    const bool last = mi + 1 >= a.m;
+    const uint32_t bk = blockIdx.x % kBuckets;
+    const uint64_t nb = (gridDim.x + kBuckets - 1 - bk) / kBuckets;  // blocks serving bucket bk
+    const uint64_t sub = blockIdx.x / kBuckets;
+    const uint64_t b0 = a.boff[bk], n = a.boff[bk + 1] - b0;
+    const uint64_t waves_total = nb * (BS / 64);
+    const uint64_t n_items = (n + GPW - 1) / GPW;
+    for (uint64_t item = sub * (BS / 64) + wave; item < n_items; item += waves_total) {
+        const uint64_t t = item * GPW + lane / SG;
+        const bool active = t < n;
+        const uint64_t i = active ? a.blist[b0 + t] : 0;
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = active ? a.st_x[i * R + g + SG * q] : 0.0;
+        wave_sync_lds();
+        double c64[EPL];
+        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                            : philox_uniform(a.seed, a.call, i, 2 * k + 1))
+                                 : 0.5;
+        double low = active ? a.st_s[i * 4 + 0] : 0.0;
+        double high = active ? a.st_s[i * 4 + 1] : 1.0;
+        const double m = active ? a.st_s[i * 4 + 2] : 1.0;
+        uint64_t c = active ? (uint64_t)a.st_s[i * 4 + 3] : ((1ull << a.split_level) - 1);
+        descend_steps<RMAX, SG, USE32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, g, m, a.split_level,
+                                       (uint32_t)md.sh.lfill_level, true, c, low, high);
+        const uint64_t idx = leaf_index(md.sh, c);
+        uint32_t res = 0;
+        if (md.sh.F > 1) {
+            const double frac = (ug - low) / (high - low);
+            const uint64_t r0 = idx * md.sh.F;
+            const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
+            if constexpr (USE32)
+                res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
+            else
+                res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
+        }
+        const uint64_t srow = idx * md.sh.F + res;
+        double h[HPL];
+#pragma unroll
+        for (int q = 0; q < HPL; q++) {
+            h[q] = 1.0;
+            if (mi > 0 && active && g + SG * q < R) h[q] = a.h_state[i * R + g + SG * q];
+            if (g + SG * q < R) h[q] *= md.U64[srow * R + g + SG * q];
+        }
+        if (active && g == 0) a.samples[(uint64_t)k * a.J + i] = srow;
+        wave_sync_lds();
+        if (!last) {
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < HPL; q++)
+                    if (g + SG * q < R) a.h_state[i * R + g + SG * q] = h[q];
+            }
+            continue;
         }
         // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
 #pragma unroll

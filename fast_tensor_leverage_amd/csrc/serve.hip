@@ -154,7 +154,7 @@ struct Server {
     }
 
     static size_t request_footprint(const Request& r) {
-        size_t b = sampler_footprint(r.N, r.I, r.R, r.precision, ArenaAllocator::kAlign);
+        size_t b = sampler_footprint(r.N, r.I, r.J, r.R, r.precision, ArenaAllocator::kAlign);
         for (uint32_t k = 0; k < r.N; k++) b += ArenaAllocator::round_up(r.I[k] * r.R * sizeof(double));
         b += ArenaAllocator::round_up(r.N * r.J * sizeof(uint64_t));
         return b;

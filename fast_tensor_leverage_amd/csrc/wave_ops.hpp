@@ -32,7 +32,8 @@ __device__ inline double dpp_f64(double v) {
 // permutations, half-row and row mirrors); across rows from ds_bpermute.
 template <int SG>
 __device__ inline double group_sum(double v) {
-    static_assert(SG >= 2 && SG <= 64 && (SG & (SG - 1)) == 0, "group width");
+    static_assert(SG >= 1 && SG <= 64 && (SG & (SG - 1)) == 0, "group width");
+    if constexpr (SG == 1) return v;
     v += dpp_f64<0xB1>(v);                     // quad_perm [1,0,3,2]
     if constexpr (SG >= 4) v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
     if constexpr (SG >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror

@@ -124,7 +124,9 @@ void* ftl_sampler_stream(ftl_sampler* s);
  * Keys: "group_lanes" (lanes per sample in the draw kernel: 8/16/32/64,
  * 0 = default), "grid_blocks" (cap on draw-kernel blocks, 0 = one block per
  * 256/group_lanes samples), "jacobi_threads" (block size of the computeM
- * eigensolver kernels: 256/512/1024, 0 = 1024), "fp32_levels" (FP32 mode:
+ * eigensolver kernels: 256/512/1024, 0 = 1024), "mode_split" (1: one draw
+ * launch per sampled mode with h carried in HBM between them, the default;
+ * 0: every mode in one launch), "fp32_levels" (FP32 mode:
  * number of bottom tree levels stored FP32; rebuilds the FP32 copies).
  * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree). */
 int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value);
