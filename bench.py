@@ -315,6 +315,82 @@ def serve_main(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# mixed precision on ill-conditioned factors (--workload c3)
+# ---------------------------------------------------------------------------
+def c3_main(args):
+    """N=3, I=2^23, J=50,000 at R in {16, 64, 128}: U_k = Q_U diag(s) Q_V^T with
+    Q_U (I x R) and Q_V (R x R) orthonormal from a seeded QR and s geometric
+    from 1 to 1e-6.  The FP64 and FP32 samplers draw on identical uniforms;
+    reported: samples/s of each, the index mismatch rate, and the relative
+    difference of the leverage probabilities of the drawn rows
+    (p = hᵀ pinv h / R = 1 / (J w))."""
+    import torch
+    sys.path.insert(0, HERE)
+    import fast_tensor_leverage_amd as ftl
+    dev = torch.device("cuda", 0)
+    N, I, J = N_MODES, args.I, J_SAMPLES
+    rows = []
+    for R in [int(r) for r in args.c3_ranks.split(",")]:
+        log(f"C3 R={R}: building ill-conditioned factors")
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(3000 + R)
+        s = torch.logspace(0, -6, R, dtype=torch.float64, device=dev)
+        U = []
+        for k in range(N):
+            qu, _ = torch.linalg.qr(torch.randn((I, R), generator=gen, device=dev, dtype=torch.float64))
+            qv, _ = torch.linalg.qr(torch.randn((R, R), generator=gen, device=dev, dtype=torch.float64))
+            U.append(((qu * s) @ qv.T).contiguous())
+            del qu, qv
+        draws = torch.rand(2 * N * J, generator=gen, device=dev, dtype=torch.float64)
+        res = {}
+        for prec in ("fp64", "fp32"):
+            smp = ftl.Sampler.from_device(U, J, R, precision=prec)
+            d_s = torch.zeros((N, J), dtype=torch.int64, device=dev)
+            d_w = torch.empty((J,), dtype=torch.float64, device=dev)
+            smp.draw_device(N + 1, d_s, None, d_w, draws)
+            torch.cuda.synchronize()
+            smp.enable_timing(True)
+            t0 = time.perf_counter()
+            for _ in range(args.c3_steps):
+                smp.draw_device(N + 1, d_s, None, d_w, draws)
+            tm = smp.last_timing()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            res[prec] = {"samples_per_s": args.c3_steps * J / el, "kernel_ms": tm,
+                         "samples": d_s.cpu().numpy().copy(), "p": (1.0 / (J * d_w)).cpu().numpy()}
+            log(f"C3 R={R} {prec}: {res[prec]['samples_per_s']:,.0f} samples/s, draw {tm['draw_ms']:.3f} ms")
+            smp.close()
+            del smp, d_s, d_w
+            torch.cuda.synchronize()
+        diff = np.any(res["fp32"]["samples"] != res["fp64"]["samples"], axis=0)
+        p32, p64 = res["fp32"]["p"], res["fp64"]["p"]
+        # rows whose leverage under the cut pinv is zero (hᵀ pinv h = 0: the
+        # row lies in the discarded eigenspace) have no relative error
+        ok = (p64 > 0) & np.isfinite(p64) & np.isfinite(p32)
+        rel = np.abs(p32[ok] - p64[ok]) / p64[ok]
+        rows.append({"R": R, "fp64_samples_per_s": res["fp64"]["samples_per_s"],
+                     "fp32_samples_per_s": res["fp32"]["samples_per_s"],
+                     "fp32_over_fp64": res["fp32"]["samples_per_s"] / res["fp64"]["samples_per_s"],
+                     "draw_ms": {"fp64": res["fp64"]["kernel_ms"]["draw_ms"],
+                                 "fp32": res["fp32"]["kernel_ms"]["draw_ms"]},
+                     "computeM_ms": res["fp64"]["kernel_ms"]["gram_pinv_ms"] + res["fp64"]["kernel_ms"]["eigen_modes_ms"],
+                     "index_mismatch_rate": float(diff.mean()),
+                     "leverage_prob_rel_diff_max": float(rel.max()) if rel.size else None,
+                     "leverage_prob_rel_diff_mean": float(rel.mean()) if rel.size else None,
+                     "drawn_rows_with_zero_leverage_fp64": float(1.0 - ok.mean())})
+        del U, draws
+        torch.cuda.empty_cache()
+    line = {"metric": "KRP samples/s, FP32 vs FP64 sampler, ill-conditioned factors (s from 1 to 1e-6)",
+            "value": rows[0]["fp32_samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.c3_steps,
+            "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (FP64 top tree levels) and f64",
+            "data": "synthetic: U = Q_U diag(geomspace(1,1e-6,R)) Q_V^T from seeded QR, uniforms seeded",
+            "config": {"workload": f"KRPDrawSamples N=3 I={I} J={J} all modes, R in {args.c3_ranks}"},
+            "ranks": rows}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,7 +402,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
-    ap.add_argument("--workload", choices=["krp", "als", "serve"], default="krp",
+    ap.add_argument("--c3-ranks", default="16,64,128")
+    ap.add_argument("--c3-steps", type=int, default=5)
+    ap.add_argument("--workload", choices=["krp", "als", "serve", "c3"], default="krp",
                     help="krp: the KRP leverage-sampling headline (default); als: sampled CP-ALS sweeps; "
                          "serve: the draw-request stream at several arrival rates")
     ap.add_argument("--als-size", choices=sorted(ALS_SIZES), default="c5")
@@ -349,6 +427,9 @@ def main():
         return
     if args.workload == "serve":
         serve_main(args)
+        return
+    if args.workload == "c3":
+        c3_main(args)
         return
 
     import torch  # device buffers only; imported before libftl_hip.so so both share one HIP runtime

@@ -92,6 +92,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m fast_tensor_leverage_amd.build`")
+    # torch ships a HIP runtime with the same soname as ROCm's; loading torch
+    # first makes libftl_hip.so bind to that one, so the process has a single
+    # HIP runtime (two of them hide the device from the second)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(L, name)

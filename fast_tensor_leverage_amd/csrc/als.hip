@@ -240,7 +240,8 @@ struct SparseDev {
         N = N_;
         nnz = nnz_;
         FTL_HIP_CHECK(hipGetDevice(&device));
-        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // blocking stream: ordered after the caller's default-stream work
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
         mt.init(mem);
         // staging copies of the input (freed at the end of construction)
         uint32_t* d_idx = nullptr;
@@ -447,7 +448,8 @@ struct Als {
         R = R_;
         if (X) FTL_REQUIRE(X->N == N, "the factor count differs from the sparse tensor's mode count");
         FTL_HIP_CHECK(hipGetDevice(&device));
-        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // blocking stream: ordered after the caller's default-stream work
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
         static bool attrs_set = false;
         if (!attrs_set) {
             // A ((R+1) R doubles) in LDS, plus V and the eigenvectors when R <= 64

@@ -303,7 +303,10 @@ struct Sampler {
             stream = use_stream;
             own_stream = false;
         } else {
-            FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            // a blocking stream: its work is ordered after the work callers
+            // queued on the legacy default stream (where torch runs by
+            // default), so device inputs written there are complete when read
+            FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
         }
         static bool attrs_set = false;
         if (!attrs_set) {

@@ -317,29 +317,35 @@ struct DrawCfg {
     static constexpr int RPP = SG / LPR;                // rows per pass
 };
 
-// Coefficients of x^T G x over the packed entries this lane owns:
-// c = x_a x_b (diagonal) or 2 x_a x_b (off-diagonal), entry e = g + SG*i.
-template <int RMAX, int SG>
-__device__ inline void make_coefs(const double* xs, int R, int g, int epl,
-                                  double (&c)[DrawCfg<RMAX, SG>::EPL]) {
-    int a = 0, off = g;
-    while (a < R && off >= R - a) {
-        off -= R - a;
-        a++;
+// (row, column) of every packed upper-triangle entry e < T, as a | b << 8
+// (R <= 128), filled once per block: make_coefs then needs two LDS reads per
+// entry instead of walking the triangle.
+template <int BS>
+__device__ inline void fill_pairs(uint16_t* pt, int R) {
+    for (int r = threadIdx.x; r < R; r += BS) {
+        const int base = r * R - r * (r - 1) / 2;
+        for (int c = r; c < R; c++) pt[base + c - r] = (uint16_t)(r | (c << 8));
     }
+    __syncthreads();
+}
+
+// Coefficients of x^T G x over the packed entries this lane owns:
+// c = x_a x_b (diagonal) or 2 x_a x_b (off-diagonal), entry e = g + SG*i,
+// zero past the packed size T.
+template <int RMAX, int SG>
+__device__ inline void make_coefs(const double* xs, const uint16_t* pt, int T, int g,
+                                  double (&c)[DrawCfg<RMAX, SG>::EPL]) {
 #pragma unroll
     for (int i = 0; i < DrawCfg<RMAX, SG>::EPL; i++) {
+        const int e = g + SG * i;
         double v = 0.0;
-        if (i < epl && a < R) {
-            const double p = xs[a] * xs[a + off];
-            v = off == 0 ? p : 2.0 * p;
+        if (e < T) {
+            const uint32_t ab = pt[e];
+            const uint32_t ra = ab & 0xffu, rb = ab >> 8;
+            const double p = xs[ra] * xs[rb];
+            v = ra == rb ? p : 2.0 * p;
         }
         c[i] = v;
-        off += SG;
-        while (a < R && off >= R - a) {
-            off -= R - a;
-            a++;
-        }
     }
 }
 
@@ -546,6 +552,8 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
     constexpr int GPW = 64 / SG;  // groups per wave
     __shared__ double lds_x[BS / SG][RMAX];
     __shared__ double lds_q[BS / SG][RMAX];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, a.R);
     const int lane = threadIdx.x & 63;
     const int g = lane % SG;
     const int wave = threadIdx.x / 64;
@@ -554,7 +562,6 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
     double* qs = lds_q[grp_in_block];
     const int R = a.R;
     const int T = (int)tri_size(R);
-    const int epl = (T + SG - 1) / SG;
     constexpr bool fp32 = USE32;
     // entries R..RMAX-1 of x stay zero (row_dot's generic path reads them)
     for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
@@ -586,7 +593,7 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 if (g + SG * q < R) xs[g + SG * q] = h[q];
             wave_sync_lds();
             double c64[EPL];
-            make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+            make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
             double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i]
                                           : philox_uniform(a.seed, a.call, i, 2 * k))
                                : 0.5;
@@ -601,7 +608,7 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 if (g + SG * q < R) xs[g + SG * q] = h[q] * md.E[r * R + g + SG * q];
             wave_sync_lds();
             // ---- Gram stage: x = scaled_h -------------------------------------
-            make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+            make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
             const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                                 : philox_uniform(a.seed, a.call, i, 2 * k + 1))
                                      : 0.5;
@@ -726,13 +733,14 @@ __global__ void __launch_bounds__(BS) k_krp_draw_wave(DrawArgs a) {
     constexpr int WPB = BS / 64;
     __shared__ double lds_x[WPB][RMAX];
     __shared__ double lds_q[WPB][RMAX];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, a.R);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     double* xs = lds_x[wave];
     double* qs = lds_q[wave];
     const int R = a.R;
     const int T = (int)tri_size(R);
-    const int epl = (T + 63) / 64;
     for (int b = lane; b < RMAX; b += 64) xs[b] = 0.0;
     wave_sync_lds();
     const uint64_t waves_total = (uint64_t)gridDim.x * WPB;
@@ -754,7 +762,7 @@ __global__ void __launch_bounds__(BS) k_krp_draw_wave(DrawArgs a) {
                 if (lane + 64 * q < R) xs[lane + 64 * q] = h[q];
             wave_sync_lds();
             double c64[EPL];
-            make_coefs<RMAX, 64>(xs, R, lane, epl, c64);
+            make_coefs<RMAX, 64>(xs, pair_tab, T, lane, c64);
             const double ue = wave_uniform(a.draws ? a.draws[(uint64_t)k * a.J + i]
                                                    : philox_uniform(a.seed, a.call, i, 2 * k));
             double low, high;
@@ -766,7 +774,7 @@ __global__ void __launch_bounds__(BS) k_krp_draw_wave(DrawArgs a) {
                 if (lane + 64 * q < R) xs[lane + 64 * q] = h[q] * md.E[r * R + lane + 64 * q];
             wave_sync_lds();
             // ---- Gram stage: x = scaled_h -----------------------------------
-            make_coefs<RMAX, 64>(xs, R, lane, epl, c64);
+            make_coefs<RMAX, 64>(xs, pair_tab, T, lane, c64);
             const double ug = wave_uniform(a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                                    : philox_uniform(a.seed, a.call, i, 2 * k + 1));
             const uint64_t c = descend_wave<RMAX, USE32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, lane, low, high);
@@ -835,13 +843,14 @@ __global__ void __launch_bounds__(BS) k_draw_prefix(DrawArgs a) {
     constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
     constexpr int GPW = 64 / SG;
     __shared__ double lds_x[BS / SG][RMAX];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, a.R);
     const int lane = threadIdx.x & 63;
     const int g = lane % SG;
     const int wave = threadIdx.x / 64;
     double* xs = lds_x[threadIdx.x / SG];
     const int R = a.R;
     const int T = (int)tri_size(R);
-    const int epl = (T + SG - 1) / SG;
     for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
     wave_sync_lds();
     const int mi = a.mi_begin;
@@ -865,7 +874,7 @@ __global__ void __launch_bounds__(BS) k_draw_prefix(DrawArgs a) {
             if (g + SG * q < R) xs[g + SG * q] = h[q];
         wave_sync_lds();
         double c64[EPL];
-        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
         const double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k))
                                  : 0.5;
         double low, high;
@@ -880,7 +889,7 @@ __global__ void __launch_bounds__(BS) k_draw_prefix(DrawArgs a) {
                 if (active) a.st_x[i * R + g + SG * q] = x;
             }
         wave_sync_lds();
-        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
         const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                             : philox_uniform(a.seed, a.call, i, 2 * k + 1))
                                  : 0.5;
@@ -952,6 +961,8 @@ __global__ void __launch_bounds__(BS) k_draw_suffix(DrawArgs a) {
     constexpr int GPW = 64 / SG;
     __shared__ double lds_x[BS / SG][RMAX];
     __shared__ double lds_q[BS / SG][RMAX];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, a.R);
     const int lane = threadIdx.x & 63;
     const int g = lane % SG;
     const int wave = threadIdx.x / 64;
@@ -959,7 +970,6 @@ __global__ void __launch_bounds__(BS) k_draw_suffix(DrawArgs a) {
     double* qs = lds_q[threadIdx.x / SG];
     const int R = a.R;
     const int T = (int)tri_size(R);
-    const int epl = (T + SG - 1) / SG;
     for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
     wave_sync_lds();
     const int mi = a.mi_begin;
@@ -981,7 +991,7 @@ __global__ void __launch_bounds__(BS) k_draw_suffix(DrawArgs a) {
             if (g + SG * q < R) xs[g + SG * q] = active ? a.st_x[i * R + g + SG * q] : 0.0;
         wave_sync_lds();
         double c64[EPL];
-        make_coefs<RMAX, SG>(xs, R, g, epl, c64);
+        make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
         const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                             : philox_uniform(a.seed, a.call, i, 2 * k + 1))
                                  : 0.5;

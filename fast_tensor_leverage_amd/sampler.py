@@ -87,6 +87,30 @@ class Sampler:
                    "Sampler")
         self._h = h
 
+    @classmethod
+    def from_device(cls, tensors, J: int, R: int, precision: str = "fp64"):
+        """Sampler over device-resident factors (torch tensors of shape I_k x R,
+        float64, contiguous, on the GPU); they are copied into the sampler."""
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        for k, t in enumerate(tensors):
+            if tuple(t.shape[1:]) != (int(R),) or t.dim() != 2:
+                raise ValueError(f"tensors[{k}] must be I_k x R")
+            if str(t.dtype) != "torch.float64":
+                raise TypeError(f"tensors[{k}] must be float64")
+        _lib.require_device()
+        obj = cls.__new__(cls)
+        obj.N, obj.J, obj.R, obj.precision = len(tensors), int(J), int(R), precision
+        obj.dims = [int(t.shape[0]) for t in tensors]
+        obj._matrices = None
+        dims = (ctypes.c_uint64 * obj.N)(*obj.dims)
+        ptrs = (ctypes.c_void_p * obj.N)(*[_dev_ptr(t, f"tensors[{k}]") for k, t in enumerate(tensors)])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().ftl_sampler_create_device(obj.N, dims, obj.J, obj.R, ptrs, PRECISIONS[precision],
+                                                         ctypes.byref(h)), "Sampler")
+        obj._h = h
+        return obj
+
     # -- lifetime ---------------------------------------------------------------
     def close(self):
         h = getattr(self, "_h", None)

@@ -5,6 +5,10 @@
  *
  * Plain C types only: host pointers are C-contiguous row-major arrays, device
  * pointers are HIP device addresses, streams are hipStream_t passed as void*.
+ * Handles run on blocking HIP streams of their own: their work is ordered
+ * after work queued earlier on the legacy default stream (torch's default),
+ * so device inputs written there are complete when read.  Inputs written on
+ * other non-blocking streams must be synchronised by the caller.
  * Every entry point returns an ftl_status; on failure ftl_last_error() returns
  * a message (thread-local) describing the cause.  No entry point exits the
  * process (the reference calls exit(1) on bad arguments, als_module.cpp:57-61).
