@@ -484,13 +484,13 @@ def main():
     bytes64 = algorithmic_bytes(I, R, J, N, 0, False)
     t_kernel = f32["timing"]["draw_ms"] * 1e-3
     achieved = bytes32 / t_kernel
-    # HBM bytes per launch of the FP32 draw kernel, from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)
+    # HBM bytes of the FP32 draw (its per-mode launches together), from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)
     traffic = None
     tfile = os.path.join(HERE, "profiles", "k_krp_draw_N3_I2p23_R32_J50000_traffic.json")
     if os.path.exists(tfile) and I == I_ROWS:
         with open(tfile) as f:
-            traffic = json.load(f)["fp32"]["hbm_bytes_per_launch"]
+            traffic = json.load(f)["fp32"]["hbm_bytes_per_draw"]
 
     serving = None
     if not args.skip_serve:
@@ -524,7 +524,8 @@ def main():
                      "traffic_GBps": (traffic / t_kernel / 1e9) if traffic else None,
                      "kernel": f"k_krp_draw<32,{f32['group_lanes']},256,fp32>",
                      "kernel_ms": f32["timing"]["draw_ms"],
-                     "algorithmic_bytes_per_launch": bytes32},
+                     "algorithmic_bytes_per_draw": bytes32,
+                     "launches_per_draw": N},
         "cpu_baseline": cpu,
         "fp64_gpu_samples_per_s": v64,
         "fp32_over_fp64": value / v64,
