@@ -72,8 +72,15 @@ struct Sampler {
     // 1: one draw launch per sampled mode (h kept in h_state between them),
     // so a launch's working set is one tree; 0: all modes in one launch
     uint32_t mode_split = 1;
-    // draw kernel: 0 = a group of lanes per sample, 1 = one sample per wave
+    // draw kernel: 0 = by rank (a group of lanes per sample for R <= 64, one
+    // sample per block above), 1 = one sample per wave (R <= 32), 2 = one
+    // sample per block (R > 32), 3 = a group of lanes per sample at any rank.
+    // Measured at N=3, I=2^23, J=5e4 (FP64 draw): R=64 group 5.68 ms, block
+    // 6.20 (128 threads) / 7.23 (256); R=128 group 172.5 ms (the 129
+    // coefficients and node values per lane spill), block 27.8 (256 threads)
     uint32_t draw_kernel = 0;
+    // threads per block of the one-sample-per-block kernel (0: by rank)
+    uint32_t block_threads = 0;
     // 1: each mode's descent bucketed by level-3 subtree onto XCD-affine
     // blocks.  Measured at N=3, I=2^23, R=32 (FP64): prefix 167 + partition
     // 97 + suffix 367 us per mode against 539 us for the one-launch mode, so
@@ -248,8 +255,12 @@ struct Sampler {
             FTL_REQUIRE(value == 0 || value == 1, "xcd_buckets must be 0 or 1");
             xcd_buckets = (uint32_t)value;
         } else if (key == "draw_kernel") {
-            FTL_REQUIRE(value == 0 || value == 1, "draw_kernel must be 0 or 1");
+            FTL_REQUIRE(value >= 0 && value <= 3, "draw_kernel must be 0, 1, 2 or 3");
             draw_kernel = (uint32_t)value;
+        } else if (key == "block_threads") {
+            FTL_REQUIRE(value == 0 || value == 128 || value == 256 || value == 512,
+                        "block_threads must be 0, 128, 256 or 512");
+            block_threads = (uint32_t)value;
         } else if (key == "mode_split") {
             FTL_REQUIRE(value == 0 || value == 1, "mode_split must be 0 or 1");
             mode_split = (uint32_t)value;
@@ -272,6 +283,7 @@ struct Sampler {
         if (key == "grid_blocks") return grid_override;
         if (key == "mode_split") return mode_split;
         if (key == "draw_kernel") return draw_kernel;
+        if (key == "block_threads") return block_threads;
         if (key == "xcd_buckets") return xcd_buckets;
         if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
@@ -518,8 +530,52 @@ struct Sampler {
         }
     }
 
+    // One-sample-per-block kernel: a grid of as many blocks as are resident
+    // at once, each looping over samples i = blockIdx.x + k * gridDim.x.
+    template <int RMAX, int BS, bool USE32>
+    void launch_draw_block(const DrawArgs& a, hipStream_t st) {
+        static int resident = 0;  // blocks per CU, per instantiation
+        static int cus = 0;
+        if (resident == 0) {
+            int dev = 0;
+            FTL_HIP_CHECK(hipGetDevice(&dev));
+            FTL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            int nb = 0;
+            FTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_krp_draw_block<RMAX, BS, USE32>, BS, 0));
+            resident = std::max(1, nb);
+        }
+        uint64_t blocks = std::min<uint64_t>(J, (uint64_t)resident * (uint64_t)std::max(1, cus));
+        if (grid_override) blocks = std::min<uint64_t>(blocks, grid_override);
+        blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 20));
+        hipLaunchKernelGGL((k_krp_draw_block<RMAX, BS, USE32>), dim3((unsigned)blocks), dim3(BS), 0, st, a);
+        FTL_HIP_CHECK(hipGetLastError());
+    }
+
+    template <int RMAX, bool USE32>
+    void launch_draw_block_bs(const DrawArgs& a, hipStream_t st) {
+        const uint32_t bs = block_threads ? block_threads : 256u;
+        if constexpr (RMAX == 64) {
+            if (bs == 128) launch_draw_block<64, 128, USE32>(a, st);
+            else launch_draw_block<64, 256, USE32>(a, st);
+        } else {
+            if (bs == 128) launch_draw_block<128, 128, USE32>(a, st);
+            else if (bs == 512) launch_draw_block<128, 512, USE32>(a, st);
+            else launch_draw_block<128, 256, USE32>(a, st);
+        }
+    }
+
     void dispatch_draw(const DrawArgs& a, hipStream_t st) {
         const uint32_t sg = sg_override;
+        if ((draw_kernel == 2 && R > 32) || (draw_kernel == 0 && R > 64)) {
+            if (R <= 64) {
+                if (precision == FTL_FP32) launch_draw_block_bs<64, true>(a, st);
+                else launch_draw_block_bs<64, false>(a, st);
+            } else {
+                if (precision == FTL_FP32) launch_draw_block_bs<128, true>(a, st);
+                else launch_draw_block_bs<128, false>(a, st);
+            }
+            return;
+        }
         if (draw_kernel == 1 && R <= 32 && sg == 0) {
             if (R <= 8) {
                 if (precision == FTL_FP32) launch_draw_wave<8, true>(a, st);
@@ -608,7 +664,8 @@ struct Sampler {
                 a.mi_begin = mi;
                 a.mi_end = mi + 1;
                 // bucketed when the tree is deep enough for the split to matter
-                if (xcd_buckets && draw_kernel == 0 && modes[a.order[mi]].sh.lfill_level >= 8) dispatch_bucketed(a, st);
+                if (xcd_buckets && draw_kernel == 0 && R <= 64 && modes[a.order[mi]].sh.lfill_level >= 8)
+                    dispatch_bucketed(a, st);
                 else dispatch_draw(a, st);
             }
         } else {

@@ -830,6 +830,260 @@ __global__ void __launch_bounds__(BS) k_krp_draw_wave(DrawArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// One-sample-per-block draw kernel for large ranks (R = 64, 128).  A packed
+// R = 128 node is 8,256 doubles: split over one group of 64 lanes it needs
+// 129 coefficients and 129 node values per lane, more than the register file
+// holds.  Here a block of BS threads (4-8 waves) works on one sample: each
+// thread owns the packed entries tid, tid + BS, ... (ceil(T/BS) of them),
+// a level sums the per-thread products with the DPP wave sum and one LDS
+// exchange between the waves (one barrier per level; the exchange slots
+// alternate between two sets so a level's writes never meet the previous
+// level's reads), and every thread then holds the same node mass, so the
+// descent branch is uniform over the block.  The leaf scan computes the F
+// row dot products with the whole block and scans them in every wave.
+// Same arithmetic and comparisons as k_krp_draw up to the order of the
+// partial sums.
+// ---------------------------------------------------------------------------
+template <int BS>
+__device__ inline double block_sum(double v, double* red, int& par) {
+    constexpr int NW = BS / 64;
+    v = group_sum<64>(v);
+    if ((threadIdx.x & 63) == 0) red[par * NW + (threadIdx.x >> 6)] = v;
+    __syncthreads();
+    double t = red[par * NW];
+#pragma unroll
+    for (int w = 1; w < NW; w++) t += red[par * NW + w];
+    par ^= 1;
+    return t;
+}
+
+template <typename TN, int EPL, int BS>
+__device__ inline double node_qform_block(const TN* __restrict__ node, const double (&c)[EPL], double* red, int& par) {
+    TN v[EPL];
+    const TN* base = node + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < EPL; i++) v[i] = base[BS * i];
+    double acc0 = 0, acc1 = 0;
+#pragma unroll
+    for (int i = 0; i < EPL; i += 2) {
+        acc0 += c[i] * (double)v[i];
+        if (i + 1 < EPL) acc1 += c[i + 1] * (double)v[i + 1];
+    }
+    return block_sum<BS>(acc0 + acc1, red, par);
+}
+
+template <int EPL, int BS, bool USE32>
+__device__ inline double child_qform_block(const ModeDev& md, const double* tree64, uint64_t child, uint32_t depth,
+                                           uint32_t ts64, uint32_t ts32, const double (&c)[EPL], double* red,
+                                           int& par) {
+    if (!USE32 || depth < md.d64) return node_qform_block<double, EPL, BS>(tree64 + child * ts64, c, red, par);
+    return node_qform_block<float, EPL, BS>(md.tree32 + (child - md.n64) * ts32, c, red, par);
+}
+
+// The inverse-CDF descent of descend (partition_tree.hpp:251-332) with
+// block-wide node masses.
+template <int EPL, int BS, bool USE32>
+__device__ inline uint64_t descend_block(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
+                                         const double (&c64)[EPL], const double* tree64, double u, double* red,
+                                         int& par, double& low, double& high) {
+    const double m = node_qform_block<double, EPL, BS>(tree64, c64, red, par);
+    uint64_t c = 0;
+    low = 0.0;
+    high = 1.0;
+    const uint32_t levels = sh.lfill_level + ((sh.node_count > sh.nodes_before_lfill) ? 1u : 0u);
+    for (uint32_t d = 0; d < levels; d++) {
+        // the guarded tail level (partition_tree.hpp:304-332): a leaf on the
+        // lfill level stops the descent
+        if (d == sh.lfill_level && 2 * c + 1 >= sh.node_count) break;
+        const uint64_t child = 2 * c + 1;
+        const double mL = child_qform_block<EPL, BS, USE32>(md, tree64, child, d + 1, ts64, ts32, c64, red, par);
+        const double cutoff = low + mL / m;
+        if (u <= cutoff) {
+            c = child;
+            high = cutoff;
+        } else {
+            c = child + 1;
+            low = cutoff;
+        }
+    }
+    return c;
+}
+
+// Leaf scan of one sample with the whole block (partition_tree.hpp:336-391):
+// LPR = RMAX/8 threads per row, 8 columns each, RPP = BS/LPR rows per pass;
+// loads for PCH passes are in flight together.  q_t = (U_t . x)^2 goes to
+// LDS, then every wave forms the same inclusive scan over the F values and
+// finds the first t < F-1 with frac < P[t+1] (else F-1).
+template <typename TU, int RMAX, int BS>
+__device__ inline uint32_t leaf_scan_block(const TU* __restrict__ U, uint64_t r0, uint32_t rows, uint32_t F, int R,
+                                           const double* xs, double* qs, double frac) {
+    constexpr int LC = 8, LPR = RMAX / LC, RPP = BS / LPR;
+    constexpr int PMAX = (RMAX + RPP - 1) / RPP;
+    constexpr int PCH = PMAX < 4 ? PMAX : 4;
+    constexpr int W = 16 / sizeof(TU);
+    typedef TU V __attribute__((ext_vector_type(W)));
+    const int tid = threadIdx.x;
+    const int sub = tid % LPR, rsub = tid / LPR;
+    const int col0 = sub * LC;
+    const uint32_t passes = (F + RPP - 1) / RPP;
+    for (uint32_t p0 = 0; p0 < passes; p0 += PCH) {
+        TU buf[PCH][LC];
+#pragma unroll
+        for (int pp = 0; pp < PCH; pp++) {
+            const uint32_t t = (p0 + pp) * RPP + rsub;
+            const uint32_t tc = t < rows ? t : rows - 1;
+            const TU* row = U + (r0 + tc) * (uint64_t)R;
+            if (R == RMAX) {
+                const V* rv = reinterpret_cast<const V*>(row + col0);
+#pragma unroll
+                for (int q = 0; q < LC / W; q++) {
+                    const V v = rv[q];
+#pragma unroll
+                    for (int w = 0; w < W; w++) buf[pp][q * W + w] = v[w];
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < LC; w++) {
+                    const int b = col0 + w;
+                    buf[pp][w] = row[b < R ? b : R - 1];
+                }
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < PCH; pp++) {
+            double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+            for (int w = 0; w < LC; w += 2) {
+                acc0 += (double)buf[pp][w] * xs[col0 + w];
+                acc1 += (double)buf[pp][w + 1] * xs[col0 + w + 1];
+            }
+            const double d = group_sum<LPR>(acc0 + acc1);
+            const uint32_t t = (p0 + pp) * RPP + rsub;
+            if (sub == 0 && t < F) qs[t] = t < rows ? d * d : 0.0;
+        }
+    }
+    __syncthreads();
+    constexpr int RPL = (RMAX + 63) / 64;
+    const int lane = tid & 63;
+    double incl[RPL];
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        if (64 * k < (int)F) {
+            const uint32_t t = lane + 64 * k;
+            const double q2 = t < F ? qs[t] : 0.0;
+            const double v = group_incl_scan<64>(q2, lane) + run;
+            incl[k] = v;
+            run = __shfl(v, 63, 64);
+        } else {
+            incl[k] = run;
+        }
+    }
+    const double total = run;
+    uint32_t res = F - 1;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        const uint32_t t = lane + 64 * k;
+        const bool pred = (t + 1 < F) && (frac < incl[k] / total);
+        const uint64_t bal = __ballot(pred);
+        if (!found && bal) {
+            res = 64 * k + __builtin_ctzll(bal);
+            found = true;
+        }
+    }
+    if (res >= rows) res = rows - 1;
+    return res;
+}
+
+template <int RMAX, int BS, bool USE32>
+__global__ void __launch_bounds__(BS) k_krp_draw_block(DrawArgs a) {
+    static_assert(BS >= RMAX && BS % 64 == 0, "one h entry per thread");
+    using Cfg = DrawCfg<RMAX, BS>;
+    constexpr int EPL = Cfg::EPL;
+    constexpr int NW = BS / 64;
+    __shared__ double xs[RMAX];
+    __shared__ double qs[RMAX];
+    __shared__ double red[2 * NW];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, a.R);
+    const int tid = threadIdx.x;
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    int par = 0;
+    // entries R..RMAX-1 of x stay zero (the leaf scan's generic path reads them)
+    for (int b = tid; b < RMAX; b += BS) xs[b] = 0.0;
+    __syncthreads();
+
+    for (uint64_t i = blockIdx.x; i < a.J; i += gridDim.x) {
+        double h = 1.0;  // thread tid < R owns h[tid]
+        if (a.mi_begin > 0 && tid < R) h = a.h_state[i * R + tid];
+        for (int mi = a.mi_begin; mi < a.mi_end; mi++) {
+            const int k = a.order[mi];
+            const ModeDev& md = a.mode[k];
+            // ---- eigen stage: x = h -----------------------------------------
+            if (tid < R) xs[tid] = h;
+            __syncthreads();
+            double c64[EPL];
+            make_coefs<RMAX, BS>(xs, pair_tab, T, tid, c64);
+            const double ue = a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k);
+            double low, high;
+            const uint64_t ce = descend_block<EPL, BS, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, red, par,
+                                                              low, high);
+            const uint64_t r = leaf_index(a.esh, ce);
+            // every thread has passed the descent's barriers after its
+            // make_coefs reads of xs
+            if (tid < R) xs[tid] = h * md.E[r * R + tid];
+            __syncthreads();
+            // ---- Gram stage: x = scaled_h -------------------------------------
+            make_coefs<RMAX, BS>(xs, pair_tab, T, tid, c64);
+            const double ug = a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                      : philox_uniform(a.seed, a.call, i, 2 * k + 1);
+            const uint64_t c = descend_block<EPL, BS, USE32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, red, par,
+                                                             low, high);
+            const uint64_t idx = leaf_index(md.sh, c);
+            uint32_t res = 0;
+            if (md.sh.F > 1) {
+                const double frac = (ug - low) / (high - low);
+                const uint64_t r0 = idx * md.sh.F;
+                const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
+                if constexpr (USE32)
+                    res = leaf_scan_block<float, RMAX, BS>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac);
+                else
+                    res = leaf_scan_block<double, RMAX, BS>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac);
+            }
+            const uint64_t srow = idx * md.sh.F + res;
+            if (tid < R) h *= md.U64[srow * R + tid];
+            if (tid == 0) a.samples[(uint64_t)k * a.J + i] = srow;
+            __syncthreads();  // xs / qs reads done before the next stage writes them
+        }
+        if (a.mi_end < a.m) {  // the next launch continues from h
+            if (tid < R) a.h_state[i * R + tid] = h;
+            continue;
+        }
+        // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122), over the
+        // packed triangle with the coefficients of x = h ----------------------
+        if (tid < R) xs[tid] = h;
+        __syncthreads();
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            const int e = tid + BS * q;
+            if (e < T) {
+                const uint32_t ab = pair_tab[e];
+                const uint32_t ra = ab & 0xffu, rb = ab >> 8;
+                const double p = xs[ra] * xs[rb] * a.pinv[ra * R + rb];
+                part += ra == rb ? p : 2.0 * p;
+            }
+        }
+        const double lev = block_sum<BS>(part, red, par);
+        if (tid == 0 && a.w_out) a.w_out[i] = (double)R / (lev * (double)a.J);
+        if (a.h_out && tid < R) a.h_out[i * R + tid] = h;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Subtree-bucketed draw of one sampled mode (positions mi_begin, one launch
 // each): k_draw_prefix runs the eigen stage and the first split_level Gram
 // levels and records the state, k_bucket_partition groups the samples by

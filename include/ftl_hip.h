@@ -131,7 +131,13 @@ void* ftl_sampler_stream(ftl_sampler* s);
  * eigensolver kernels: 256/512/1024, 0 = 1024), "mode_split" (1: one draw
  * launch per sampled mode with h carried in HBM between them, the default;
  * 0: every mode in one launch), "fp32_levels" (FP32 mode:
- * number of bottom tree levels stored FP32; rebuilds the FP32 copies).
+ * number of bottom tree levels stored FP32; rebuilds the FP32 copies),
+ * "draw_kernel" (0 = by rank: a group of lanes per sample for R <= 64, one
+ * sample per block above; 1 = one sample per wave, R <= 32; 2 = one sample
+ * per block, R > 32; 3 = a group of lanes per sample at any rank),
+ * "block_threads" (threads per block of the one-sample-per-block kernel:
+ * 128/256/512, 0 = 256), "xcd_buckets" (1: descents bucketed by level-3
+ * subtree onto XCD-affine blocks, R <= 64).
  * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree). */
 int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value);
 int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value);

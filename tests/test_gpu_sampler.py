@@ -79,10 +79,53 @@ def test_fp64_parity_ill_conditioned(ftl, oracle, cid, dims, R, J, j):
     _check_fp64_parity(ftl, oracle, cid, U, R, J, j, check_e_rows=False)
 
 
-def _check_fp64_parity(ftl, oracle, cid, U, R, J, j, check_e_rows):
+# one-sample-per-block draw kernel (draw_kernel=2) at R > 32: id, dims, R, J,
+# j, threads per block
+BLOCK_CASES = [
+    ("blk_r64_excl0", [2048, 1000, 3000], 64, 4_000, 0, 256),
+    ("blk_r64_bs128", [2048, 1000, 3000], 64, 4_000, 4, 128),
+    ("blk_r48_short_mode", [1500, 900, 47], 48, 3_000, 4, 256),
+    ("blk_r128", [700, 300], 128, 1_000, 2, 256),
+    ("blk_r100_bs512", [900, 400, 333], 100, 1_500, 1, 512),
+    ("blk_r128_bs128", [1200, 500], 128, 1_000, 2, 128),
+]
+
+
+@pytest.mark.parametrize("cid,dims,R,J,j,bs", BLOCK_CASES, ids=[c[0] for c in BLOCK_CASES])
+def test_fp64_parity_block_kernel(ftl, oracle, cid, dims, R, J, j, bs):
+    print(f"[gpu] fp64 parity block kernel case={cid} dims={dims} R={R} J={J} j={j} bs={bs}", flush=True)
+    U = _factors(dims, R, seed=zlib.crc32(cid.encode()))
+    _check_fp64_parity(ftl, oracle, cid, U, R, J, j, check_e_rows=True,
+                       options={"draw_kernel": 2, "block_threads": bs})
+
+
+@pytest.mark.parametrize("cid,dims,R,J,j,bs", BLOCK_CASES, ids=[c[0] for c in BLOCK_CASES])
+def test_fp32_mismatch_rate_block_kernel(ftl, oracle, cid, dims, R, J, j, bs):
+    print(f"[gpu] fp32 block kernel case={cid} bs={bs}", flush=True)
+    U = _factors(dims, R, seed=zlib.crc32(cid.encode()))
+    N = len(dims)
+    ref = oracle.OracleSampler(U, J, R)
+    gpu = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    gpu.set_option("draw_kernel", 2)
+    gpu.set_option("block_threads", bs)
+    draws = np.random.default_rng(11).random(2 * N * J)
+    out = ref.draw(j, draws)
+    samples = np.zeros((N, J), dtype=np.uint64)
+    gpu.KRPDrawSamples(j, samples, draws)
+    diff = np.zeros(J, dtype=bool)
+    for k in range(N):
+        if not (j < N and k == j):
+            diff |= samples[k] != out["samples"][k]
+    print(f"[gpu] {cid}: fp32 mismatch rate {diff.mean():.2e}", flush=True)
+    assert diff.mean() <= 2e-3
+
+
+def _check_fp64_parity(ftl, oracle, cid, U, R, J, j, check_e_rows, options=None):
     N = len(U)
     ref = oracle.OracleSampler(U, J, R)
     gpu = ftl.Sampler(U, J, R, "efficient", precision="fp64")
+    for key, val in (options or {}).items():
+        gpu.set_option(key, val)
     # trees: bit-exact
     for k in range(N):
         np.testing.assert_array_equal(gpu.get_tree(k), ref.tree(k))
