@@ -62,6 +62,25 @@ struct Sampler {
     uint32_t fp32_levels = kDefaultFp32Levels;
     uint64_t tree_pad = 0;  // zeroed entries after every tree (draw kernel over-read)
     std::vector<ModeHost> modes;
+    // computeM outputs and scratch.  A sampler that owns its stream keeps two
+    // sets and runs the computeM of call n on cm_stream into set n % 2 while
+    // the draw kernels of call n-1 (set (n-1) % 2) still run on the draw
+    // stream: cm_done orders a set's draw after its computeM, draw_done the
+    // next computeM into the set after the draw that read it, and trees_ev
+    // every computeM after the tree builds queued before it.
+    struct MSet {
+        double *pinv = nullptr, *W = nullptr, *E = nullptr, *etree = nullptr, *work = nullptr, *lam = nullptr;
+        hipEvent_t cm_done = nullptr, draw_done = nullptr;
+    };
+    MSet msets[2];
+    uint32_t n_msets = 1;
+    uint64_t mset_next = 0;
+    hipStream_t cm_stream = nullptr;
+    hipEvent_t trees_ev = nullptr;
+    bool trees_dirty = true;
+    // option "overlap_compute_m" (only with a stream of the sampler's own)
+    uint32_t overlap_cm = 1;
+    // set 0 (compute_m_host and the single-set samplers read these)
     double *pinv = nullptr, *W = nullptr, *E = nullptr, *etree = nullptr, *work = nullptr, *lam = nullptr;
     uint32_t etree_nodes = 0;
     // staging for the host-pointer API
@@ -92,8 +111,8 @@ struct Sampler {
     double* h_state = nullptr;
     bool timing = false;
     // events of every timed call since the last read-out: {start, after
-    // gram_pinv, after eigen_modes, after draw}
-    std::vector<std::array<hipEvent_t, 4>> pending;
+    // gram_pinv, after eigen_modes, draw start, draw end}
+    std::vector<std::array<hipEvent_t, 5>> pending;
     std::vector<hipEvent_t> event_pool;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double last_ms[3] = {0, 0, 0};
@@ -114,10 +133,11 @@ struct Sampler {
     void collect_timing() {
         double acc[3] = {0, 0, 0};
         for (auto& q : pending) {
-            FTL_HIP_CHECK(hipEventSynchronize(q[3]));
+            FTL_HIP_CHECK(hipEventSynchronize(q[4]));
+            const int from[3] = {0, 1, 3}, to[3] = {1, 2, 4};
             for (int i = 0; i < 3; i++) {
                 float t = 0;
-                FTL_HIP_CHECK(hipEventElapsedTime(&t, q[i], q[i + 1]));
+                FTL_HIP_CHECK(hipEventElapsedTime(&t, q[from[i]], q[to[i]]));
                 acc[i] += t;
             }
             for (auto e : q) event_pool.push_back(e);
@@ -164,7 +184,14 @@ struct Sampler {
     }
 
     ~Sampler() {
+        if (cm_stream) (void)hipStreamSynchronize(cm_stream);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& ms : msets) {
+            if (ms.cm_done) (void)hipEventDestroy(ms.cm_done);
+            if (ms.draw_done) (void)hipEventDestroy(ms.draw_done);
+        }
+        if (trees_ev) (void)hipEventDestroy(trees_ev);
+        if (cm_stream) (void)hipStreamDestroy(cm_stream);
         for (auto& a : allocs) release_block(a.first, a.second);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -187,8 +214,11 @@ struct Sampler {
         m.n64 = std::min<uint64_t>(full, m.sh.node_count);
     }
 
+    // Builds on a stream other than the sampler's own occur only for samplers
+    // on a borrowed stream, which keep one computeM set on that stream.
     void build_mode(uint32_t k, hipStream_t st = nullptr) {
         if (!st) st = stream;
+        trees_dirty = true;
         ModeHost& m = modes[k];
         BuildArgs ba;
         ba.U = m.U64;
@@ -264,6 +294,10 @@ struct Sampler {
         } else if (key == "mode_split") {
             FTL_REQUIRE(value == 0 || value == 1, "mode_split must be 0 or 1");
             mode_split = (uint32_t)value;
+        } else if (key == "overlap_compute_m") {
+            FTL_REQUIRE(value == 0 || value == 1, "overlap_compute_m must be 0 or 1");
+            if (cm_stream) FTL_HIP_CHECK(hipStreamSynchronize(cm_stream));
+            overlap_cm = (uint32_t)value;
         } else if (key == "jacobi_threads") {
             FTL_REQUIRE(value == 0 || value == 256 || value == 512 || value == 1024,
                         "jacobi_threads must be 0, 256, 512 or 1024");
@@ -284,6 +318,7 @@ struct Sampler {
         if (key == "mode_split") return mode_split;
         if (key == "draw_kernel") return draw_kernel;
         if (key == "block_threads") return block_threads;
+        if (key == "overlap_compute_m") return overlap_cm && n_msets > 1;
         if (key == "xcd_buckets") return xcd_buckets;
         if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
@@ -319,6 +354,12 @@ struct Sampler {
             // queued on the legacy default stream (where torch runs by
             // default), so device inputs written there are complete when read
             FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
+            // computeM reads only the tree roots (ordered by trees_ev) and
+            // writes its own buffer set, so its stream need not follow the
+            // legacy default stream
+            FTL_HIP_CHECK(hipStreamCreateWithFlags(&cm_stream, hipStreamNonBlocking));
+            FTL_HIP_CHECK(hipEventCreateWithFlags(&trees_ev, hipEventDisableTiming));
+            n_msets = 2;
         }
         static bool attrs_set = false;
         if (!attrs_set) {
@@ -370,20 +411,37 @@ struct Sampler {
         }
         const uint64_t R2 = R * R;
         etree_nodes = (uint32_t)(2 * R - 1);
-        pinv = dalloc<double>(R2);
-        W = dalloc<double>(R2);
-        E = dalloc<double>(N * R2);
-        etree = dalloc<double>((uint64_t)N * etree_nodes * ts64 + pad);
-        work = dalloc<double>((uint64_t)(N + 1) * 3 * (R + 1) * R);
-        lam = dalloc<double>((uint64_t)(N + 1) * R);
+        for (uint32_t q = 0; q < n_msets; q++) {
+            MSet& ms = msets[q];
+            ms.pinv = dalloc<double>(R2);
+            ms.W = dalloc<double>(R2);
+            ms.E = dalloc<double>(N * R2);
+            ms.etree = dalloc<double>((uint64_t)N * etree_nodes * ts64 + pad);
+            ms.work = dalloc<double>((uint64_t)(N + 1) * 3 * (R + 1) * R);
+            ms.lam = dalloc<double>((uint64_t)(N + 1) * R);
+            FTL_HIP_CHECK(hipMemsetAsync(ms.E, 0, N * R2 * sizeof(double), stream));
+            FTL_HIP_CHECK(
+                hipMemsetAsync(ms.etree, 0, ((uint64_t)N * etree_nodes * ts64 + pad) * sizeof(double), stream));
+            if (n_msets > 1) {
+                FTL_HIP_CHECK(hipEventCreateWithFlags(&ms.cm_done, hipEventDisableTiming));
+                FTL_HIP_CHECK(hipEventCreateWithFlags(&ms.draw_done, hipEventDisableTiming));
+                // the first computeM into the set waits for the zero fills
+                FTL_HIP_CHECK(hipEventRecord(ms.draw_done, stream));
+            }
+        }
+        pinv = msets[0].pinv;
+        W = msets[0].W;
+        E = msets[0].E;
+        etree = msets[0].etree;
+        work = msets[0].work;
+        lam = msets[0].lam;
         h_state = dalloc<double>(J * R);
         st_x = dalloc<double>(J * R);
         st_s = dalloc<double>(J * 4);
         bucket = dalloc<uint32_t>(J);
         blist = dalloc<uint32_t>(J);
         boff = dalloc<uint32_t>(kBuckets + 1);
-        FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R2 * sizeof(double), stream));
-        FTL_HIP_CHECK(hipMemsetAsync(etree, 0, ((uint64_t)N * etree_nodes * ts64 + pad) * sizeof(double), stream));
+        trees_dirty = true;
         if (sync_end) FTL_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -436,27 +494,27 @@ struct Sampler {
         return out;
     }
 
-    ComputeMArgs computeM_args(const std::vector<int>& order) {
+    ComputeMArgs computeM_args(const std::vector<int>& order, const MSet& ms) {
         ComputeMArgs a{};
         for (uint32_t k = 0; k < N; k++) a.roots[k] = modes[k].tree64;
         for (size_t t = 0; t < order.size(); t++) a.modes[t] = order[t];
         a.m = (int)order.size();
         a.R = (int)R;
-        a.pinv = pinv;
-        a.W = W;
-        a.E = E;
-        a.etree = etree;
-        a.work = work;
-        a.lam_out = lam;
+        a.pinv = ms.pinv;
+        a.W = ms.W;
+        a.E = ms.E;
+        a.etree = ms.etree;
+        a.work = ms.work;
+        a.lam_out = ms.lam;
         a.ts64 = ts64;
         a.etree_nodes = etree_nodes;
         return a;
     }
 
-    void compute_m(uint32_t j, hipStream_t st, hipEvent_t* evs = nullptr) {
+    void compute_m(uint32_t j, hipStream_t st, const MSet& ms, hipEvent_t* evs = nullptr) {
         std::vector<int> order = sampled_modes(j);
         FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
-        ComputeMArgs a = computeM_args(order);
+        ComputeMArgs a = computeM_args(order, ms);
         const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
         const size_t lds = jac * sizeof(double);
         const uint32_t bs = jacobi_override ? jacobi_override : jacobi_threads_for(R);
@@ -619,10 +677,26 @@ struct Sampler {
         FTL_REQUIRE(samples != nullptr, "null samples buffer");
         if (!st) st = stream;
         std::vector<int> order = sampled_modes(j);
-        std::array<hipEvent_t, 4> evs{};
+        FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
+        std::array<hipEvent_t, 5> evs{};
         if (timing)
             for (auto& e : evs) e = take_event();
-        compute_m(j, st, timing ? evs.data() : nullptr);
+        const bool overlap = overlap_cm && n_msets > 1;
+        const MSet& ms = msets[overlap ? (mset_next++ & 1) : 0];
+        if (overlap) {
+            if (trees_dirty) {
+                FTL_HIP_CHECK(hipEventRecord(trees_ev, stream));
+                trees_dirty = false;
+            }
+            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, trees_ev, 0));
+            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, ms.draw_done, 0));
+            compute_m(j, cm_stream, ms, timing ? evs.data() : nullptr);
+            FTL_HIP_CHECK(hipEventRecord(ms.cm_done, cm_stream));
+            FTL_HIP_CHECK(hipStreamWaitEvent(st, ms.cm_done, 0));
+        } else {
+            compute_m(j, st, ms, timing ? evs.data() : nullptr);
+        }
+        if (timing) FTL_HIP_CHECK(hipEventRecord(evs[3], st));
         DrawArgs a{};
         for (uint32_t k = 0; k < N; k++) {
             const ModeHost& m = modes[k];
@@ -631,8 +705,8 @@ struct Sampler {
             d.tree32 = m.tree32;
             d.U64 = m.U64;
             d.U32 = m.U32;
-            d.etree = etree + (uint64_t)k * etree_nodes * ts64;
-            d.E = E + (uint64_t)k * R * R;
+            d.etree = ms.etree + (uint64_t)k * etree_nodes * ts64;
+            d.E = ms.E + (uint64_t)k * R * R;
             d.sh = m.sh;
             d.n64 = m.n64;
             d.d64 = m.d64;
@@ -645,7 +719,7 @@ struct Sampler {
         a.ts64 = ts64;
         a.ts32 = ts32;
         a.esh = make_shape(R, 1);
-        a.pinv = pinv;
+        a.pinv = ms.pinv;
         a.draws = draws;
         a.seed = seed;
         a.call = call;
@@ -674,8 +748,9 @@ struct Sampler {
             a.h_state = nullptr;
             dispatch_draw(a, st);
         }
+        if (overlap) FTL_HIP_CHECK(hipEventRecord(ms.draw_done, st));
         if (timing) {
-            FTL_HIP_CHECK(hipEventRecord(evs[3], st));
+            FTL_HIP_CHECK(hipEventRecord(evs[4], st));
             pending.push_back(evs);
         }
         if (!draws) call++;
@@ -724,8 +799,9 @@ struct Sampler {
     }
 
     void compute_m_host(uint32_t j, double* pinv_out, double* E_out) {
+        if (cm_stream) FTL_HIP_CHECK(hipStreamSynchronize(cm_stream));
         FTL_HIP_CHECK(hipMemsetAsync(E, 0, N * R * R * sizeof(double), stream));
-        compute_m(j, stream);
+        compute_m(j, stream, msets[0]);
         if (pinv_out)
             FTL_HIP_CHECK(hipMemcpyAsync(pinv_out, pinv, R * R * sizeof(double), hipMemcpyDeviceToHost, stream));
         if (E_out)

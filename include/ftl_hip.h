@@ -137,16 +137,21 @@ void* ftl_sampler_stream(ftl_sampler* s);
  * per block, R > 32; 3 = a group of lanes per sample at any rank),
  * "block_threads" (threads per block of the one-sample-per-block kernel:
  * 128/256/512, 0 = 256), "xcd_buckets" (1: descents bucketed by level-3
- * subtree onto XCD-affine blocks, R <= 64).
+ * subtree onto XCD-affine blocks, R <= 64), "overlap_compute_m" (1, the
+ * default for a sampler with a stream of its own: computeM of a call runs on
+ * a second stream into one of two buffer sets, overlapping the previous
+ * call's draw kernels; 0: computeM and draw in order on one stream).
  * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree). */
 int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value);
 int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value);
 
-/* Kernel timing with HIP events recorded on the launch stream.  While enabled,
- * every draw records four events without synchronising; last_timing waits for
- * the recorded calls and returns average durations (ms) over them:
- * out[0] = k_gram_pinv, out[1] = k_eigen_modes, out[2] = k_krp_draw,
- * out[3] = number of calls averaged.  Reading clears the record. */
+/* Kernel timing with HIP events recorded on the launch streams.  While
+ * enabled, every draw records five events without synchronising (computeM
+ * start, after k_gram_pinv, after k_eigen_modes on the computeM stream; draw
+ * start and end on the draw stream); last_timing waits for the recorded calls
+ * and returns average durations (ms) over them: out[0] = k_gram_pinv,
+ * out[1] = k_eigen_modes, out[2] = the draw kernels, out[3] = number of calls
+ * averaged.  Reading clears the record. */
 int ftl_sampler_enable_timing(ftl_sampler* s, int enable);
 int ftl_sampler_last_timing(ftl_sampler* s, double* out4);
 

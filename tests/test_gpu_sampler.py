@@ -272,3 +272,37 @@ def test_device_factors_ordered_after_default_stream(ftl):
     s_dev.KRPDrawSamples(N + 1, a, draws)
     s_host.KRPDrawSamples(N + 1, b, draws)
     np.testing.assert_array_equal(a, b)
+
+
+def test_overlapped_compute_m_matches_serial(ftl):
+    """computeM of a call runs on a side stream into one of two buffer sets
+    while the previous call's draw kernels run; back-to-back device calls
+    (changing j, a factor updated in between) give the same samples and
+    weights as the serial order."""
+    import torch
+    dev = torch.device("cuda", 0)
+    dims, R, J = [3000, 2000, 1500], 16, 4000
+    N = len(dims)
+    U = _factors(dims, R, 5)
+    new = np.random.default_rng(6).standard_normal((dims[1], R))
+    js = [4, 0, 1, 4, 2, 4, 1]
+    draws = [torch.from_numpy(np.random.default_rng(100 + t).random(2 * N * J)).to(dev) for t in range(len(js))]
+    outs = {}
+    for overlap in (1, 0):
+        print(f"[gpu] overlapped computeM: overlap_compute_m={overlap}", flush=True)
+        s = ftl.Sampler(U, J, R)
+        s.set_option("overlap_compute_m", overlap)
+        assert s.get_option("overlap_compute_m") == overlap
+        res = [torch.zeros((N, J), dtype=torch.int64, device=dev) for _ in js]
+        wts = [torch.zeros(J, dtype=torch.float64, device=dev) for _ in js]
+        for t, j in enumerate(js):
+            s.draw_device(j, res[t], None, wts[t], draws[t])
+            if t == 3:
+                s.update_sampler(1, new)
+        torch.cuda.synchronize()
+        outs[overlap] = ([r.cpu().numpy() for r in res], [w.cpu().numpy() for w in wts])
+        s.close()
+    for a, b in zip(outs[1][0], outs[0][0]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(outs[1][1], outs[0][1]):
+        np.testing.assert_array_equal(a, b)
