@@ -25,6 +25,11 @@ namespace {
 // Levels below this many from the bottom are stored FP32 in FP32 mode.
 constexpr uint32_t kDefaultFp32Levels = 6;
 
+// Zeroed entries after the last node of every tree beyond one packed node:
+// covers the over-read of the widest draw kernel (pairs of entries over 512
+// lanes per sample, 1024 entries) with a margin.
+constexpr uint64_t kTreePadExtra = 1088;
+
 struct ModeHost {
     uint64_t I = 0, F = 0;
     Shape sh{};
@@ -374,12 +379,13 @@ struct Sampler {
                 FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
             attrs_set = true;
         }
-        // Tail pad of every tree: the draw kernel reads SG * ceil(Tmax/SG)
-        // entries from a node start, Tmax being the packed size for the
-        // kernel's compile-time rank bound (R rounded up to a power of two).
+        // Tail pad of every tree: the draw kernels read 2 SG ceil(Tmax/(2 SG))
+        // entries from a node start (SG up to 512 lanes per sample), Tmax
+        // being the packed size for the kernel's compile-time rank bound (R
+        // rounded up to a power of two).
         uint64_t rmax = 8;
         while (rmax < R) rmax *= 2;
-        const uint64_t pad = tri_size(rmax) + 64;
+        const uint64_t pad = tri_size(rmax) + kTreePadExtra;
         tree_pad = pad;
         modes.resize(N);
         for (uint32_t k = 0; k < N; k++) {
@@ -453,7 +459,7 @@ struct Sampler {
         const uint64_t ts64 = tri_stride64(R), ts32 = tri_stride32(R);
         uint64_t rmax = 8;
         while (rmax < R) rmax *= 2;
-        const uint64_t pad = tri_size(rmax) + 64;
+        const uint64_t pad = tri_size(rmax) + kTreePadExtra;
         size_t total = 0;
         for (uint32_t k = 0; k < N; k++) {
             ModeHost m;

@@ -308,7 +308,11 @@ constexpr uint32_t kBucketLevel = 3;
 template <int RMAX, int SG>
 struct DrawCfg {
     static constexpr int TMAX = RMAX * (RMAX + 1) / 2;
-    static constexpr int EPL = (TMAX + SG - 1) / SG;   // packed entries per lane
+    // a lane reads VEC consecutive packed entries per load: entries
+    // VEC (g + SG i) + v, v < VEC, for i < NLD
+    static constexpr int VEC = 2;
+    static constexpr int NLD = (TMAX + VEC * SG - 1) / (VEC * SG);  // loads per lane per node
+    static constexpr int EPL = VEC * NLD;                            // packed entries per lane
     static constexpr int HPL = (RMAX + SG - 1) / SG;   // vector entries per lane
     static constexpr int RPL = (RMAX + SG - 1) / SG;   // leaf rows per lane (scan)
     // leaf dot products: LPR lanes per row, 8 columns per lane
@@ -330,44 +334,52 @@ __device__ inline void fill_pairs(uint16_t* pt, int R) {
 }
 
 // Coefficients of x^T G x over the packed entries this lane owns:
-// c = x_a x_b (diagonal) or 2 x_a x_b (off-diagonal), entry e = g + SG*i,
-// zero past the packed size T.
+// c = x_a x_b (diagonal) or 2 x_a x_b (off-diagonal); c[VEC i + v] belongs to
+// entry VEC (g + SG i) + v, zero past the packed size T.
 template <int RMAX, int SG>
 __device__ inline void make_coefs(const double* xs, const uint16_t* pt, int T, int g,
                                   double (&c)[DrawCfg<RMAX, SG>::EPL]) {
+    using Cfg = DrawCfg<RMAX, SG>;
 #pragma unroll
-    for (int i = 0; i < DrawCfg<RMAX, SG>::EPL; i++) {
-        const int e = g + SG * i;
-        double v = 0.0;
-        if (e < T) {
-            const uint32_t ab = pt[e];
-            const uint32_t ra = ab & 0xffu, rb = ab >> 8;
-            const double p = xs[ra] * xs[rb];
-            v = ra == rb ? p : 2.0 * p;
+    for (int i = 0; i < Cfg::NLD; i++) {
+#pragma unroll
+        for (int v = 0; v < Cfg::VEC; v++) {
+            const int e = Cfg::VEC * (g + SG * i) + v;
+            double val = 0.0;
+            if (e < T) {
+                const uint32_t ab = pt[e];
+                const uint32_t ra = ab & 0xffu, rb = ab >> 8;
+                const double p = xs[ra] * xs[rb];
+                val = ra == rb ? p : 2.0 * p;
+            }
+            c[Cfg::VEC * i + v] = val;
         }
-        c[i] = v;
     }
 }
 
-// x^T G x over one packed node.  The EPL loads per lane are unconditional and
-// use one base address with immediate offsets, so all of them are in flight
-// together.  Entries past the packed size T (the node stride is rounded to
-// 16/32 entries, which SG * EPL can exceed) belong to the next node or to the
-// zeroed tail pad every tree allocation carries; their coefficients are zero
-// and the values are finite, so they add nothing.
+// x^T G x over one packed node.  The EPL/2 paired loads per lane are
+// unconditional and use one base address with immediate offsets, so all of
+// them are in flight together.  Entries past the packed size T (the node
+// stride is rounded to 16/32 entries, which SG * EPL can exceed) belong to the
+// next node or to the zeroed tail pad every tree allocation carries; their
+// coefficients are zero and the values are finite, so they add nothing.
 // Nodes of FP32 levels are widened to FP64 before the multiply-add, so the
 // only FP32 error is the storage rounding of each entry.
 template <typename TN, int EPL, int SG>
 __device__ inline double node_qform(const TN* __restrict__ node, const double (&c)[EPL], int g) {
-    TN v[EPL];
-    const TN* base = node + g;
+    // pairs of entries: one 8-byte (FP32) or 16-byte (FP64) load each; node
+    // starts are multiples of 16 (FP64) / 32 (FP32) entries
+    typedef TN V2 __attribute__((ext_vector_type(2)));
+    constexpr int NLD = EPL / 2;
+    V2 v[NLD];
+    const V2* base = reinterpret_cast<const V2*>(node) + g;
 #pragma unroll
-    for (int i = 0; i < EPL; i++) v[i] = base[SG * i];
+    for (int i = 0; i < NLD; i++) v[i] = base[SG * i];
     double acc0 = 0, acc1 = 0;
 #pragma unroll
-    for (int i = 0; i < EPL; i += 2) {
-        acc0 += c[i] * (double)v[i];
-        if (i + 1 < EPL) acc1 += c[i + 1] * (double)v[i + 1];
+    for (int i = 0; i < NLD; i++) {
+        acc0 += c[2 * i] * (double)v[i].x;
+        acc1 += c[2 * i + 1] * (double)v[i].y;
     }
     return group_sum<SG>(acc0 + acc1);
 }
@@ -859,15 +871,17 @@ __device__ inline double block_sum(double v, double* red, int& par) {
 
 template <typename TN, int EPL, int BS>
 __device__ inline double node_qform_block(const TN* __restrict__ node, const double (&c)[EPL], double* red, int& par) {
-    TN v[EPL];
-    const TN* base = node + threadIdx.x;
+    typedef TN V2 __attribute__((ext_vector_type(2)));
+    constexpr int NLD = EPL / 2;
+    V2 v[NLD];
+    const V2* base = reinterpret_cast<const V2*>(node) + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < EPL; i++) v[i] = base[BS * i];
+    for (int i = 0; i < NLD; i++) v[i] = base[BS * i];
     double acc0 = 0, acc1 = 0;
 #pragma unroll
-    for (int i = 0; i < EPL; i += 2) {
-        acc0 += c[i] * (double)v[i];
-        if (i + 1 < EPL) acc1 += c[i + 1] * (double)v[i + 1];
+    for (int i = 0; i < NLD; i++) {
+        acc0 += c[2 * i] * (double)v[i].x;
+        acc1 += c[2 * i + 1] * (double)v[i].y;
     }
     return block_sum<BS>(acc0 + acc1, red, par);
 }
