@@ -716,6 +716,7 @@ struct Sampler {
             d.sh = m.sh;
             d.n64 = m.n64;
             d.d64 = m.d64;
+            d.T = (uint32_t)tri_size(R);
         }
         for (size_t t = 0; t < order.size(); t++) a.order[t] = order[t];
         a.m = (int)order.size();

@@ -265,6 +265,7 @@ struct ModeDev {
     Shape sh;
     uint64_t n64;          // first FP32 node (= node_count in FP64 mode)
     uint32_t d64;          // levels [0, d64) are FP64
+    uint32_t T;            // packed entries per node, R (R + 1) / 2
 };
 
 struct DrawArgs {
@@ -366,7 +367,7 @@ __device__ inline void make_coefs(const double* xs, const uint16_t* pt, int T, i
 // Nodes of FP32 levels are widened to FP64 before the multiply-add, so the
 // only FP32 error is the storage rounding of each entry.
 template <typename TN, int EPL, int SG>
-__device__ inline double node_qform(const TN* __restrict__ node, const double (&c)[EPL], int g) {
+__device__ inline double node_qform(const TN* __restrict__ node, const double (&c)[EPL], int g, uint32_t T) {
     // pairs of entries: one 8-byte (FP32) or 16-byte (FP64) load each; node
     // starts are multiples of 16 (FP64) / 32 (FP32) entries
     typedef TN V2 __attribute__((ext_vector_type(2)));
@@ -374,7 +375,10 @@ __device__ inline double node_qform(const TN* __restrict__ node, const double (&
     V2 v[NLD];
     const V2* base = reinterpret_cast<const V2*>(node) + g;
 #pragma unroll
-    for (int i = 0; i < NLD; i++) v[i] = base[SG * i];
+    for (int i = 0; i + 1 < NLD; i++) v[i] = base[SG * i];
+    // the last pair of a lane past the node's T entries re-reads its first
+    // pair (same cache lines, coefficient zero) instead of the next node
+    v[NLD - 1] = *(2u * (uint32_t)(g + SG * (NLD - 1)) < T ? base + SG * (NLD - 1) : base);
     double acc0 = 0, acc1 = 0;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
@@ -389,8 +393,8 @@ __device__ inline double node_qform(const TN* __restrict__ node, const double (&
 template <int EPL, int SG, bool USE32>
 __device__ inline double child_qform(const ModeDev& md, const double* tree64, uint64_t child, uint32_t depth,
                                      uint32_t ts64, uint32_t ts32, const double (&c)[EPL], int g) {
-    if (!USE32 || depth < md.d64) return node_qform<double, EPL, SG>(tree64 + child * ts64, c, g);
-    return node_qform<float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c, g);
+    if (!USE32 || depth < md.d64) return node_qform<double, EPL, SG>(tree64 + child * ts64, c, g, md.T);
+    return node_qform<float, EPL, SG>(md.tree32 + (child - md.n64) * ts32, c, g, md.T);
 }
 
 // One inverse-CDF descent (partition_tree.hpp:251-332) for the group's sample.
@@ -399,7 +403,7 @@ __device__ inline uint64_t descend(const ModeDev& md, const Shape& sh, uint32_t 
                                    const double (&c64)[DrawCfg<RMAX, SG>::EPL], const double* tree64, double u,
                                    int g, double& low, double& high) {
     constexpr int EPL = DrawCfg<RMAX, SG>::EPL;
-    const double m = node_qform<double, EPL, SG>(tree64, c64, g);
+    const double m = node_qform<double, EPL, SG>(tree64, c64, g, md.T);
     uint64_t c = 0;
     low = 0.0;
     high = 1.0;
@@ -702,7 +706,7 @@ __device__ inline uint64_t descend_wave(const ModeDev& md, const Shape& sh, uint
                                         const double (&c64)[DrawCfg<RMAX, 64>::EPL], const double* tree64, double u,
                                         int g, double& low, double& high) {
     constexpr int EPL = DrawCfg<RMAX, 64>::EPL;
-    const double m = wave_uniform(node_qform<double, EPL, 64>(tree64, c64, g));
+    const double m = wave_uniform(node_qform<double, EPL, 64>(tree64, c64, g, md.T));
     uint64_t c = 0;
     double lo = 0.0, hi = 1.0;
     for (uint32_t d = 0; d < sh.lfill_level; d++) {
@@ -870,13 +874,15 @@ __device__ inline double block_sum(double v, double* red, int& par) {
 }
 
 template <typename TN, int EPL, int BS>
-__device__ inline double node_qform_block(const TN* __restrict__ node, const double (&c)[EPL], double* red, int& par) {
+__device__ inline double node_qform_block(const TN* __restrict__ node, const double (&c)[EPL], double* red, int& par,
+                                          uint32_t T) {
     typedef TN V2 __attribute__((ext_vector_type(2)));
     constexpr int NLD = EPL / 2;
     V2 v[NLD];
     const V2* base = reinterpret_cast<const V2*>(node) + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < NLD; i++) v[i] = base[BS * i];
+    for (int i = 0; i + 1 < NLD; i++) v[i] = base[BS * i];
+    v[NLD - 1] = *(2u * (uint32_t)(threadIdx.x + BS * (NLD - 1)) < T ? base + BS * (NLD - 1) : base);
     double acc0 = 0, acc1 = 0;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
@@ -890,8 +896,8 @@ template <int EPL, int BS, bool USE32>
 __device__ inline double child_qform_block(const ModeDev& md, const double* tree64, uint64_t child, uint32_t depth,
                                            uint32_t ts64, uint32_t ts32, const double (&c)[EPL], double* red,
                                            int& par) {
-    if (!USE32 || depth < md.d64) return node_qform_block<double, EPL, BS>(tree64 + child * ts64, c, red, par);
-    return node_qform_block<float, EPL, BS>(md.tree32 + (child - md.n64) * ts32, c, red, par);
+    if (!USE32 || depth < md.d64) return node_qform_block<double, EPL, BS>(tree64 + child * ts64, c, red, par, md.T);
+    return node_qform_block<float, EPL, BS>(md.tree32 + (child - md.n64) * ts32, c, red, par, md.T);
 }
 
 // The inverse-CDF descent of descend (partition_tree.hpp:251-332) with
@@ -900,7 +906,7 @@ template <int EPL, int BS, bool USE32>
 __device__ inline uint64_t descend_block(const ModeDev& md, const Shape& sh, uint32_t ts64, uint32_t ts32,
                                          const double (&c64)[EPL], const double* tree64, double u, double* red,
                                          int& par, double& low, double& high) {
-    const double m = node_qform_block<double, EPL, BS>(tree64, c64, red, par);
+    const double m = node_qform_block<double, EPL, BS>(tree64, c64, red, par, md.T);
     uint64_t c = 0;
     low = 0.0;
     high = 1.0;
@@ -1161,7 +1167,7 @@ __global__ void __launch_bounds__(BS) k_draw_prefix(DrawArgs a) {
         const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
                                             : philox_uniform(a.seed, a.call, i, 2 * k + 1))
                                  : 0.5;
-        const double m = node_qform<double, EPL, SG>(md.tree64, c64, g);
+        const double m = node_qform<double, EPL, SG>(md.tree64, c64, g, md.T);
         uint64_t c = 0;
         low = 0.0;
         high = 1.0;
