@@ -497,34 +497,44 @@ __device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint
     const int sub = g % LPR, rsub = g / LPR;
     const int col0 = sub * LC;
     const uint32_t passes = (F + RPP - 1) / RPP;
-    for (uint32_t p = 0; p < passes; p++) {
-        const uint32_t t = p * RPP + rsub;
-        const uint32_t tc = t < rows ? t : rows - 1;
-        const TU* row = U + (r0 + tc) * (uint64_t)R;
-        TU buf[LC];
-        if (R == RMAX) {
-            const V* rv = reinterpret_cast<const V*>(row + col0);
+    // the loads of up to PCH passes are in flight together
+    constexpr int PMAX = (RMAX + RPP - 1) / RPP;
+    constexpr int PCH = PMAX < 4 ? PMAX : 4;
+    for (uint32_t p0 = 0; p0 < passes; p0 += PCH) {
+        TU buf[PCH][LC];
 #pragma unroll
-            for (int q = 0; q < LC / W; q++) {
-                const V v = rv[q];
+        for (int pp = 0; pp < PCH; pp++) {
+            const uint32_t t = (p0 + pp) * RPP + rsub;
+            const uint32_t tc = t < rows ? t : rows - 1;
+            const TU* row = U + (r0 + tc) * (uint64_t)R;
+            if (R == RMAX) {
+                const V* rv = reinterpret_cast<const V*>(row + col0);
 #pragma unroll
-                for (int w = 0; w < W; w++) buf[q * W + w] = v[w];
-            }
-        } else {
+                for (int q = 0; q < LC / W; q++) {
+                    const V v = rv[q];
 #pragma unroll
-            for (int w = 0; w < LC; w++) {
-                const int b = col0 + w;
-                buf[w] = row[b < R ? b : R - 1];
+                    for (int w = 0; w < W; w++) buf[pp][q * W + w] = v[w];
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < LC; w++) {
+                    const int b = col0 + w;
+                    buf[pp][w] = row[b < R ? b : R - 1];
+                }
             }
         }
-        double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
-        for (int w = 0; w < LC; w += 2) {
-            acc0 += (double)buf[w] * xs[col0 + w];
-            acc1 += (double)buf[w + 1] * xs[col0 + w + 1];
+        for (int pp = 0; pp < PCH; pp++) {
+            double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+            for (int w = 0; w < LC; w += 2) {
+                acc0 += (double)buf[pp][w] * xs[col0 + w];
+                acc1 += (double)buf[pp][w + 1] * xs[col0 + w + 1];
+            }
+            const double d = group_sum<LPR>(acc0 + acc1);
+            const uint32_t t = (p0 + pp) * RPP + rsub;
+            if (sub == 0 && t < F) qs[t] = t < rows ? d * d : 0.0;
         }
-        const double d = group_sum<LPR>(acc0 + acc1);
-        if (sub == 0 && t < F) qs[t] = t < rows ? d * d : 0.0;
     }
     wave_sync_lds();
     double incl[RPL];
