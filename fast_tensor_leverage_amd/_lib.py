@@ -54,6 +54,7 @@ SIGNATURES = {
     "ftl_als_set_seed": ([_vp, _u64, _u64], _i32),
     "ftl_als_ds_update": ([_vp, _u32, _i32, _i32, _vp], _i32),
     "ftl_als_ds_update_device": ([_vp, _u32, _i32, _i32, _vp], _i32),
+    "ftl_als_exact_update": ([_vp, _u32, _i32, _i32], _i32),
     "ftl_als_get_factor": ([_vp, _u32, _vp], _i32),
     "ftl_als_set_factor": ([_vp, _u32, _vp], _i32),
     "ftl_als_update_sampler": ([_vp, _u32], _i32),

@@ -291,7 +291,17 @@ class ALS:
             self._pull_sigma()
 
     def execute_exact_als_update(self, j: int, renormalize: bool = True, update_sampler: bool = True):
-        raise NotImplementedError("the exact (unsampled) ALS update is not on the sampled path")
+        """ALS::execute_exact_als_update (als.hpp:222-270): U_j = (exact
+        MTTKRP) pinv((*)_{k != j} U_k^T U_k), then renormalize_columns(j).
+        update_sampler is accepted and unused, as in the reference."""
+        if not 0 <= int(j) < self.cp_decomp.N:
+            raise ValueError("j out of range")
+        L = _lib.load()
+        _lib.check(L.ftl_als_exact_update(self._fh._h, int(j), int(bool(renormalize)), int(bool(update_sampler))),
+                   "execute_exact_als_update")
+        if self.host_sync:
+            _lib.check(L.ftl_als_get_factor(self._fh._h, j, _ptr(self.cp_decomp.U[j])))
+            self._pull_sigma()
 
     def _pull_sigma(self):
         _lib.check(_lib.load().ftl_als_get_sigma(self._fh._h, _ptr(self.cp_decomp.sigma)))

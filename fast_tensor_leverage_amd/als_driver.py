@@ -10,19 +10,19 @@ from .als import ALS
 
 def als_prod(lhs, rhs, J, method, max_iter, stop_tolerance, epoch_length=5, verbose=False,
              precision="fp64", seed=None, host_sync=True):
-    """Sweeps of ``execute_ds_als_update(j, True, True)`` over the modes, an
-    exact fit every ``epoch_length`` sweeps, stopped when the best of the last
-    three fits does not beat the best before them by ``stop_tolerance``
-    (als.py:99-101).  Returns the reference's record (iterations, fits,
-    update_times, fit_computation_times)."""
+    """Sweeps of ``execute_ds_als_update(j, True, True)`` (method
+    "efficient") or ``execute_exact_als_update(j, True, True)`` (method
+    "exact") over the modes, an exact fit every ``epoch_length`` sweeps,
+    stopped when the best of the last three fits does not beat the best before
+    them by ``stop_tolerance`` (als.py:59-107).  Returns the reference's record
+    (iterations, fits, update_times, fit_computation_times)."""
     def verb_print(arg):
         if verbose:
             print(arg)
 
-    if method == "exact":
-        raise NotImplementedError("the exact ALS update is not on the sampled path")
     als = ALS(lhs.ten, rhs.ten, host_sync=host_sync)
-    als.initialize_ds_als(J, method, precision=precision, seed=seed)
+    if method != "exact":
+        als.initialize_ds_als(J, method, precision=precision, seed=seed)
 
     iterations, fits, update_times, fit_computation_times = [0], [lhs.compute_exact_fit(rhs)], [], []
     verb_print(f"Before ALS:\tFit: {fits[-1]}")
@@ -30,7 +30,10 @@ def als_prod(lhs, rhs, J, method, max_iter, stop_tolerance, epoch_length=5, verb
         verb_print(f"Starting Iteration {i + 1}.")
         for j in range(lhs.N):
             start = time.time()
-            als.execute_ds_als_update(j, True, True)
+            if method == "exact":
+                als.execute_exact_als_update(j, True, True)
+            else:
+                als.execute_ds_als_update(j, True, True)
             update_times.append(time.time() - start)
         if (i + 1) % epoch_length == 0:
             start = time.time()

@@ -224,6 +224,15 @@ struct SparseDev {
     Matches mt;
     uint32_t* tup = nullptr;
     uint64_t tup_cap = 0;
+    // exact MTTKRP of mode j: lookup-j positions sorted stably by the mode-j
+    // coordinate, the row segments (D rows) and their reduction pieces (W),
+    // built on first use
+    struct RowOrder {
+        uint32_t *pos = nullptr, *seg_start = nullptr, *seg_row = nullptr, *piece_off = nullptr;
+        uint32_t* counters = nullptr;  // kCntSlots, kCntRows = D, kCntPieces = W
+        uint64_t D = 0, W = 0;
+    };
+    RowOrder rows_of[kMaxModes];
 
     ~SparseDev() {
         if (stream) {
@@ -316,6 +325,57 @@ struct SparseDev {
             tup_cap = n;
         }
         return tup;
+    }
+
+    // The row order of mode j (sort_lookup.hpp:199-254 visits the nonzeros in
+    // lookup-j order and scatters to rows; here the same nonzeros are grouped
+    // by row, keeping the lookup order within a row, so that each row is one
+    // deterministic segmented sum).  One host wait, once per mode.
+    const RowOrder& row_order(uint32_t j) {
+        RowOrder& o = rows_of[j];
+        if (o.pos) return o;
+        Arena stage;
+        StreamDrain drain{stream};
+        uint32_t* iota = stage.alloc<uint32_t>(nnz);
+        uint32_t* srow = stage.alloc<uint32_t>(nnz);
+        uint32_t* flags = stage.alloc<uint32_t>(nnz);
+        uint32_t* segid = stage.alloc<uint32_t>(nnz);
+        uint32_t* start = stage.alloc<uint32_t>(nnz + 1);
+        uint32_t* srows = stage.alloc<uint32_t>(nnz);
+        uint32_t* npieces = stage.alloc<uint32_t>(nnz);
+        uint32_t* poff = stage.alloc<uint32_t>(nnz);
+        o.pos = mem.alloc<uint32_t>(nnz);
+        o.counters = mem.alloc<uint32_t>(kCntSlots);
+        FTL_HIP_CHECK(hipMemsetAsync(o.counters, 0, kCntSlots * sizeof(uint32_t), stream));
+        const unsigned g = grid_for(nnz, 256, 8192);
+        hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, stream, iota, nnz);
+        FTL_LAUNCH_CHECK();
+        sort_pairs(cub, row[j], srow, iota, o.pos, nnz, bits[j], stream);
+        hipLaunchKernelGGL(k_head_flags<uint32_t>, dim3(g), dim3(256), 0, stream, srow, nnz, flags);
+        FTL_LAUNCH_CHECK();
+        inclusive_sum(cub, flags, segid, nnz, stream);
+        hipLaunchKernelGGL(k_seg_heads, dim3(g), dim3(256), 0, stream, srow, segid, nnz, start, srows, o.counters);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_seg_pieces, dim3(g), dim3(256), 0, stream, start, o.counters, nnz, npieces);
+        FTL_LAUNCH_CHECK();
+        exclusive_sum(cub, npieces, poff, nnz, stream);
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, stream, poff, npieces, nnz, o.counters,
+                           (int)kCntPieces);
+        FTL_LAUNCH_CHECK();
+        uint32_t hc[kCntSlots];
+        FTL_HIP_CHECK(hipMemcpyAsync(hc, o.counters, sizeof(hc), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        o.D = hc[kCntRows];
+        o.W = hc[kCntPieces];
+        o.seg_start = mem.alloc<uint32_t>(o.D + 1);
+        o.seg_row = mem.alloc<uint32_t>(o.D);
+        o.piece_off = mem.alloc<uint32_t>(o.D);
+        FTL_HIP_CHECK(hipMemcpyAsync(o.seg_start, start, (o.D + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                     stream));
+        FTL_HIP_CHECK(hipMemcpyAsync(o.seg_row, srows, o.D * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        FTL_HIP_CHECK(hipMemcpyAsync(o.piece_off, poff, o.D * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        return o;
+        // stage's buffers are freed after the stream drained
     }
 
     ProbeArgs probe_args(uint32_t j, const uint32_t* t, const uint32_t* n_dev, uint64_t n_host, uint64_t n_cap,
@@ -425,6 +485,10 @@ struct Als {
     uint32_t *vals_a = nullptr, *vals_b = nullptr, *flags = nullptr, *uid = nullptr;
     uint32_t *utup = nullptr, *ufirst = nullptr, *ustart = nullptr;
     double *hs = nullptr, *hdd = nullptr, *gpart = nullptr, *pinv = nullptr, *work = nullptr, *lam = nullptr;
+    // exact update state
+    double *gram_had = nullptr, *exact_partial = nullptr;
+    uint64_t exact_partial_cap = 0;
+    uint32_t* full_rank = nullptr;  // counters with kCntUnique = R (pinv at full target rank)
     // diagnostics
     bool timing = false;
     std::array<hipEvent_t, 5> ev{};
@@ -613,6 +677,82 @@ struct Als {
         record(4);
     }
 
+    // execute_exact_als_update(j, renormalize, update_sampler) (als.hpp:222-270):
+    // pinv of (*)_{k != j} U_k^T U_k at full target rank (compute_pinv_square,
+    // common.h:228-269), U_j = exact MTTKRP * pinv on the rows that hold
+    // nonzeros (zero elsewhere), then renormalize_columns(j).  update_sampler
+    // is accepted and unused, as in the reference.
+    template <int G, int CPL>
+    unsigned launch_exact(const SparseDev::RowOrder& o, uint32_t j, double* colsq) {
+        constexpr int GPB = 256 / G;
+        FactorPtrs F{};
+        for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
+        hipLaunchKernelGGL((k_exact_item_partial<G, CPL>), dim3(grid_for(o.W, GPB, 16384)), dim3(256), 0, stream,
+                           o.seg_start, o.piece_off, o.counters, o.pos, X->keys[j], X->val[j], X->layout[j], F,
+                           (uint32_t)R, exact_partial);
+        FTL_LAUNCH_CHECK();
+        const unsigned fblocks = grid_for(o.D, GPB, kSegBlocksMax);
+        hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(fblocks), dim3(256), 0, stream, o.seg_row, o.piece_off,
+                           o.counters, exact_partial, (uint32_t)R, pinv, U[j], colsq);
+        FTL_LAUNCH_CHECK();
+        return fblocks;
+    }
+
+    void exact_update(uint32_t j, bool renormalize, bool /*update_sampler*/) {
+        FTL_REQUIRE(X, "the exact update needs a sparse tensor");
+        FTL_REQUIRE(j < N, "mode index out of range");
+        const SparseDev::RowOrder& o = X->row_order(j);
+        // X's stream built the row order; this handle's stream reads it
+        FTL_HIP_CHECK(hipStreamSynchronize(X->stream));
+        const uint64_t T = R * (R + 1) / 2;
+        if (!pinv) {
+            pinv = mem.alloc<double>(R * R);
+            work = mem.alloc<double>(3 * (R + 1) * R);
+            lam = mem.alloc<double>(R);
+        }
+        if (!gram_had) {
+            gram_had = mem.alloc<double>(T);
+            full_rank = mem.alloc<uint32_t>(kCntSlots);
+            const uint32_t fr[kCntSlots] = {(uint32_t)R, 0, 0, 0};
+            FTL_HIP_CHECK(hipMemcpyAsync(full_rank, fr, sizeof(fr), hipMemcpyHostToDevice, stream));
+            FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        }
+        if (o.W * R > exact_partial_cap) {
+            mem.release(exact_partial);
+            exact_partial = mem.alloc<double>(o.W * R);
+            exact_partial_cap = o.W * R;
+        }
+        hipStream_t st = stream;
+        factor_grams();
+        hipLaunchKernelGGL(k_gram_hadamard, dim3(grid_for(T, 256)), dim3(256), 0, st, gram_red, N, (uint32_t)T, j,
+                           gram_had);
+        FTL_LAUNCH_CHECK();
+        const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
+        if (jacobi_threads_for(R) == 256)
+            hipLaunchKernelGGL(k_pinv_partials<256>, dim3(1), dim3(256), jac * sizeof(double), st, gram_had, 1u,
+                               full_rank, (uint32_t)R, pinv, work, lam);
+        else
+            hipLaunchKernelGGL(k_pinv_partials<1024>, dim3(1), dim3(1024), jac * sizeof(double), st, gram_had, 1u,
+                               full_rank, (uint32_t)R, pinv, work, lam);
+        FTL_LAUNCH_CHECK();
+        FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), st));
+        double* colsq = renormalize ? colpart : nullptr;
+        unsigned parts;
+        if (R <= 32) parts = launch_exact<32, 1>(o, j, colsq);
+        else if (R <= 64) parts = launch_exact<64, 1>(o, j, colsq);
+        else parts = launch_exact<64, 2>(o, j, colsq);
+        if (renormalize) {
+            hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(256), 0, st, colpart, parts, (uint32_t)R, norms, sigma, 0);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_scale_rows, dim3(grid_for(o.D * R, 256)), dim3(256), 0, st, U[j], o.seg_row,
+                               o.counters + kCntRows, (uint64_t)0, (uint32_t)R, norms);
+            FTL_LAUNCH_CHECK();
+        }
+        last_unique = 0;
+        last_found = X->nnz;
+        have_events = false;
+    }
+
     // renormalize_columns(j); j = -1: every mode, sigma = product of norms
     void renormalize(int j) {
         FTL_REQUIRE(j >= -1 && j < (int)N, "mode index out of range");
@@ -636,8 +776,8 @@ struct Als {
         FTL_HIP_CHECK(hipStreamSynchronize(st));
     }
 
-    // ||[[sigma; U]]||^2 into scal[1] (enqueued)
-    void lowrank_normsq_async() {
+    // packed U_k^T U_k of every mode into gram_red (enqueued)
+    void factor_grams() {
         const uint64_t T = R * (R + 1) / 2;
         uint64_t imax = 1;
         for (uint32_t k = 0; k < N; k++) imax = std::max(imax, I[k]);
@@ -646,6 +786,11 @@ struct Als {
         hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(N * T, 256)), dim3(256), 0, stream, gram_parts, nb,
                            (uint32_t)T, N, gram_red);
         FTL_LAUNCH_CHECK();
+    }
+
+    // ||[[sigma; U]]||^2 into scal[1] (enqueued)
+    void lowrank_normsq_async() {
+        factor_grams();
         hipLaunchKernelGGL(k_chain_final, dim3(1), dim3(256), 0, stream, gram_red, N, (uint32_t)R, sigma, scal + 1);
         FTL_LAUNCH_CHECK();
     }
@@ -899,6 +1044,14 @@ int ftl_als_ds_update(ftl_als* a, uint32_t j, int renormalize, int update_sample
         }
         A.ds_update(j, renormalize != 0, update_sampler != 0, d);
         FTL_HIP_CHECK(hipStreamSynchronize(A.stream));
+    });
+}
+
+int ftl_als_exact_update(ftl_als* a, uint32_t j, int renormalize, int update_sampler) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        a->impl.exact_update(j, renormalize != 0, update_sampler != 0);
+        FTL_HIP_CHECK(hipStreamSynchronize(a->impl.stream));
     });
 }
 

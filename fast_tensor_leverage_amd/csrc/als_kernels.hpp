@@ -71,6 +71,12 @@ __device__ inline uint64_t pack_tuple(const KeyLayout& L, const uint32_t* t) {
     return key;
 }
 
+// 64-bit lane broadcast within a group of `width` lanes
+__device__ inline uint64_t shfl_u64(uint64_t v, int src, int width) {
+    const uint32_t lo = __shfl((uint32_t)v, src, width), hi = __shfl((uint32_t)(v >> 32), src, width);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline uint32_t key_field(const KeyLayout& L, uint64_t key, uint32_t k) {
     const uint32_t b = L.bits[k];
     return b == 0 ? 0u : (uint32_t)((key >> L.shift[k]) & (b >= 64 ? ~0ull : ((1ull << b) - 1)));
@@ -478,6 +484,88 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
     }
 }
 
+// Piece w of the exact MTTKRP (sort_lookup.hpp:199-254): the nonzeros of
+// one output row, in the order of the mode-j lookup (pos[] lists lookup
+// positions sorted stably by the mode-j coordinate), each adding
+// prod_{k != j} U_k[i_k] * value -- the product formed from 1 over the modes
+// in order, then times the value, as the reference's partial_prod -- summed
+// sequentially per piece.  Keys and values of G nonzeros are fetched at once
+// (one per lane) and broadcast; lane c holds column c (CPL per lane).
+template <int G, int CPL>
+__global__ void __launch_bounds__(256) k_exact_item_partial(const uint32_t* __restrict__ seg_start,
+                                                            const uint32_t* __restrict__ piece_off,
+                                                            const uint32_t* __restrict__ counters,
+                                                            const uint32_t* __restrict__ pos,
+                                                            const uint64_t* __restrict__ keys,
+                                                            const double* __restrict__ vals, KeyLayout L,
+                                                            FactorPtrs F, uint32_t R, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t lane = threadIdx.x % G;
+    const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
+    for (uint64_t w = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; w < W; w += groups) {
+        uint32_t l = 0, h = D;  // segment of piece w: largest d with piece_off[d] <= w
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (piece_off[mid] <= w) l = mid;
+            else h = mid;
+        }
+        const uint64_t i0 = (uint64_t)seg_start[l] + (uint64_t)(w - piece_off[l]) * kSegPiece;
+        const uint64_t i1 = min<uint64_t>(i0 + kSegPiece, seg_start[l + 1]);
+        double acc[CPL];
+#pragma unroll
+        for (int q = 0; q < CPL; q++) acc[q] = 0.0;
+        for (uint64_t c0 = i0; c0 < i1; c0 += G) {
+            const int cnt = (int)min<uint64_t>(G, i1 - c0);
+            uint64_t my_key = 0;
+            double my_v = 0.0;
+            if ((int)lane < cnt) {
+                const uint32_t p = pos[c0 + lane];
+                my_key = keys[p];
+                my_v = vals[p];
+            }
+            for (int t = 0; t < cnt; t++) {
+                const uint64_t key = shfl_u64(my_key, t, G);
+                const double v = __shfl(my_v, t, G);
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    if (c < R) {
+                        double pr = 1.0;
+                        for (uint32_t k = 0; k < L.N; k++)
+                            if (k != L.skip) pr = pr * F.U[k][(uint64_t)key_field(L, key, k) * R + c];
+                        acc[q] = acc[q] + pr * v;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const uint32_t c = lane + q * G;
+            if (c < R) partial[(uint64_t)w * R + c] = acc[q];
+        }
+    }
+}
+
+// Packed Gram of the exact update: (*)_{k != skip} U_k^T U_k from the
+// per-mode packed Grams, starting from ones and multiplying in mode order
+// (ATB_chain_prod with unit sigmas, common.h:152-213).
+__global__ void k_gram_hadamard(const double* __restrict__ G, uint32_t N, uint32_t T, uint32_t skip,
+                                double* __restrict__ out) {
+#pragma clang fp contract(off)
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < T; e += gridDim.x * blockDim.x) {
+        double p = 1.0;
+        for (uint32_t k = 0; k < N; k++)
+            if (k != skip) p = p * G[(uint64_t)k * T + e];
+        out[e] = p;
+    }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)i;
+}
+
 // Segment d: sum of its pieces in order, then either the row itself (plain
 // MTTKRP) or its product with pinv (the ALS solve U_j = MTTKRP pinv,
 // als.hpp:189-203), written to row seg_row[d] of out.  With colsq given, the
@@ -648,11 +736,6 @@ __global__ void k_fill(double* __restrict__ p, uint64_t n, double v) {
 // run of G consecutive nonzeros: their keys, coordinates and values are
 // fetched one per lane and broadcast; lane r holds column r (CPL per lane).
 // ---------------------------------------------------------------------------
-__device__ inline uint64_t shfl_u64(uint64_t v, int src, int width) {
-    const uint32_t lo = __shfl((uint32_t)v, src, width), hi = __shfl((uint32_t)(v >> 32), src, width);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 template <int G, int CPL, int BS>
 __global__ void __launch_bounds__(BS) k_residual_partial(const uint64_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ rows,

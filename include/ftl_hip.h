@@ -254,6 +254,18 @@ int ftl_als_ds_update(ftl_als* a, uint32_t j, int renormalize, int update_sample
 int ftl_als_ds_update_device(ftl_als* a, uint32_t j, int renormalize, int update_sampler,
                              const double* draws_dev);
 
+/*
+ * execute_exact_als_update(j, renormalize, update_sampler) -- replaces
+ * ALS::execute_exact_als_update (als.hpp:222-270): gram = Hadamard product of
+ * U_k^T U_k over k != j, pinv = compute_pinv_square(gram, R) (cut 1e-11),
+ * U_j = exact MTTKRP over every nonzero (sort_lookup.hpp:199-254) times pinv,
+ * optional renormalize_columns(j).  update_sampler is accepted and unused, as
+ * in the reference.  Needs no initialize_ds.  The first call for a mode sorts
+ * the nonzeros by their mode-j coordinate (kept for later calls).  Returns
+ * when done.
+ */
+int ftl_als_exact_update(ftl_als* a, uint32_t j, int renormalize, int update_sampler);
+
 /* Factor k to / from host memory (I[k] x R FP64).  set_factor does not
  * rebuild the sampler's tree: call ftl_als_update_sampler (update_sampler,
  * efficient_krp_sampler.hpp:73-75). */

@@ -320,4 +320,66 @@ int ftlo_als_update(void* sampler, void* sparse, double* const* U, const uint64_
     return 0;
 }
 
+// SortIdxLookup::execute_exact_mttkrp through lookups[j] (sparse_tensor.hpp:79-82,
+// sort_lookup.hpp:199-254): result (Ij x R, zero-filled here) gets, for every
+// nonzero z in lookup-j order, value_z * prod_{k != j} U_k[z_k] added to row
+// z_j.  The product starts from 1 and takes the modes in order, then the
+// value (the reference's partial_prod, reused along runs of equal
+// coordinates off mode j, which gives the same numbers).  One sequential
+// pass: the reference's OpenMP chunks and atomics fix no order.
+void ftlo_sparse_exact_mttkrp(void* h, uint64_t j, const double* const* U, uint64_t R, uint64_t Ij, double* result) {
+    Sparse* S = (Sparse*)h;
+    const uint64_t N = S->N;
+    const Lookup& L = S->lookups[j];
+    std::fill(result, result + Ij * R, 0.0);
+    std::vector<double> pp(R);
+    for (uint64_t i = 0; i < S->nnz; i++) {
+        const uint64_t z = L.order[i];
+        const uint32_t* id = S->idx + z * N;
+        for (uint64_t u = 0; u < R; u++) pp[u] = 1.0;
+        for (uint64_t k = 0; k < N; k++) {
+            if (k == j) continue;
+            for (uint64_t u = 0; u < R; u++) pp[u] *= U[k][(uint64_t)id[k] * R + u];
+        }
+        const double x = S->val[z];
+        double* out = result + (uint64_t)id[j] * R;
+        for (uint64_t u = 0; u < R; u++) out[u] += pp[u] * x;
+    }
+}
+
+// ALS::execute_exact_als_update(j, renormalize, update_sampler) (als.hpp:222-270):
+// gram = (*)_{k != j} U_k^T U_k (ATB_chain_prod with unit sigmas, common.h:152-213:
+// start from ones, multiply by each mode's Gram in mode order), pinv =
+// compute_pinv_square(gram, R) (1e-11 cut, full target rank), U_j = exact
+// MTTKRP * pinv (cblas_dsymm), then renormalize_columns(j) (sigma <- norms).
+// U[j] is overwritten in place.
+int ftlo_als_exact_update(void* sparse, double* const* U, const uint64_t* dims, uint64_t N, uint64_t R, uint64_t j,
+                          int renormalize, double* sigma) {
+    std::vector<double> gram(R * R, 1.0), G(R * R);
+    for (uint64_t k = 0; k < N; k++) {
+        if (k == j) continue;
+        std::fill(G.begin(), G.end(), 0.0);
+        for (uint64_t i = 0; i < dims[k]; i++)
+            for (uint64_t a = 0; a < R; a++)
+                for (uint64_t b = 0; b < R; b++) G[a * R + b] += U[k][i * R + a] * U[k][i * R + b];
+        for (uint64_t e = 0; e < R * R; e++) gram[e] *= G[e];
+    }
+    std::vector<double> pinv(R * R);
+    pinv_square(gram.data(), R, R, pinv.data());
+    const uint64_t Ij = dims[j];
+    std::vector<double> mt(Ij * R);
+    ftlo_sparse_exact_mttkrp(sparse, j, U, R, Ij, mt.data());
+    double* Uj = U[j];
+    #pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < Ij; i++)
+        for (uint64_t c = 0; c < R; c++) {
+            double acc = 0.0;
+            for (uint64_t k = 0; k < R; k++) acc += mt[i * R + k] * pinv[k * R + c];
+            Uj[i * R + c] = acc;
+        }
+    if (renormalize) ftlo_renormalize_columns(Uj, Ij, R, sigma);
+    return 0;
+}
+
 }  // extern "C"
+

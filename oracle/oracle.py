@@ -37,6 +37,12 @@ built here: it needs cblas.h/lapacke.h):
                           (2) the weighted sampled least-squares solution
                           computed densely from the same samples, h, weights,
                           (3) sweeps drive the fit of a noisy tensor to ~1
+  exact_mttkrp            the dense MTTKRP X_(j) (KRP of the other factors)
+                          of the densified tensor (numpy einsum)
+  als_exact_update        the dense least-squares solution argmin ||X_(j) -
+                          U_j KRP^T|| (numpy lstsq); exact recovery of a
+                          noise-free factor; the fit never decreases over a
+                          sweep (ALS is monotone on the full problem)
 """
 from __future__ import annotations
 
@@ -112,6 +118,9 @@ def _als_bind(L):
     L.ftlo_als_update.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp, ctypes.c_int, ctypes.c_int,
                                   _vp, _vp]
     L.ftlo_als_update.restype = ctypes.c_int
+    L.ftlo_sparse_exact_mttkrp.argtypes = [_vp, _u64, _vp, _u64, _u64, _vp]
+    L.ftlo_als_exact_update.argtypes = [_vp, _vp, _vp, _u64, _u64, _u64, ctypes.c_int, _vp]
+    L.ftlo_als_exact_update.restype = ctypes.c_int
 
 
 def _p(a):
@@ -305,6 +314,16 @@ class OracleSparse:
         dims = np.array([u.shape[0] for u in U], dtype=np.uint64)
         return float(lib().ftlo_sparse_residual_normsq(self._h, _p(sigma), _ptrs(U), _p(dims), U[0].shape[1]))
 
+    def exact_mttkrp(self, j, U_list):
+        """execute_exact_mttkrp(U, j) (sparse_tensor.hpp:79-82): Ij x R with
+        Ij = U_list[j].shape[0]."""
+        U = [_c(u, np.float64) for u in U_list]
+        R = U[0].shape[1]
+        Ij = U[j].shape[0]
+        out = np.empty((Ij, R))
+        lib().ftlo_sparse_exact_mttkrp(self._h, j, _ptrs(U), R, Ij, _p(out))
+        return out
+
 
 def lowrank_normsq(U_list, sigma) -> float:
     U = [_c(u, np.float64) for u in U_list]
@@ -375,6 +394,15 @@ class OracleALS:
         if rc != 0:
             raise RuntimeError("oracle ALS update failed")
         return {"unique_samples": int(diag[0]), "found": int(diag[1])}
+
+    def exact_update(self, j, renormalize=True, update_sampler=True):
+        """execute_exact_als_update (als.hpp:222-270).  update_sampler is
+        accepted and unused, as in the reference."""
+        dims = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
+        rc = lib().ftlo_als_exact_update(self.X._h, _ptrs(self.U), _p(dims), self.N, self.R, j, int(renormalize),
+                                         _p(self.sigma))
+        if rc != 0:
+            raise RuntimeError("oracle exact ALS update failed")
 
     def residual_normsq(self) -> float:
         return self.X.residual_normsq(self.sigma, self.U)

@@ -230,3 +230,89 @@ def test_als_errors(ftl):
     # a factor with fewer rows than the tensor's mode size is refused
     with pytest.raises(ValueError):
         ftl.ALS(ftl.LowRankTensor(3, [rng.standard_normal((d, 3)) for d in (4, 6, 7)]), X)
+
+
+# ---- exact (unsampled) update: execute_exact_als_update (als.hpp:222-270) ----
+
+EXACT_CASES = [
+    # id, dims, R, nnz, skewed (a full slice at mode-0 index 0: a row of 320 nonzeros)
+    ("x_r4", (30, 25, 20), 4, 2000, False),
+    ("x_skew_r8", (40, 20, 16), 8, 1500, True),
+    ("x_n4_r3", (12, 10, 9, 8), 3, 2500, False),
+    ("x_r40", (50, 45, 40), 40, 6000, False),
+    ("x_r70", (60, 50, 80), 70, 8000, False),
+    ("x_n2_r6", (300, 250), 6, 5000, False),
+]
+
+
+@pytest.mark.parametrize("cid,dims,R,nnz,skewed", EXACT_CASES, ids=[c[0] for c in EXACT_CASES])
+def test_exact_updates_match_oracle(ftl, oracle, cid, dims, R, nnz, skewed):
+    """U_j within 1e-9 relative of the oracle (the MTTKRP is summed per row in
+    pieces and the pinv's eigensolver rounds differently), sigma within 1e-10,
+    rows of mode j without nonzeros exactly zero."""
+    print(f"[gpu] exact ALS parity case={cid}", flush=True)
+    rng = np.random.default_rng(_seed(cid))
+    N = len(dims)
+    idx, vals = _skewed_sparse(dims, nnz, rng) if skewed else random_sparse(dims, nnz, rng, unique=False)
+    U0 = [rng.standard_normal((d, R)) for d in dims]
+    ref = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U0], 16)
+    lr = ftl.LowRankTensor(R, [u.copy() for u in U0])
+    als = ftl.ALS(lr, ftl.SparseTensor(idx, vals))
+    for step, j in enumerate([0, 1, N - 1, 0]):
+        renorm = step != 1
+        ref.exact_update(j, renormalize=renorm)
+        als.execute_exact_als_update(j, renorm, True)
+        scale = np.abs(ref.U[j]).max()
+        np.testing.assert_allclose(lr.U[j], ref.U[j], rtol=1e-9, atol=1e-11 * scale, err_msg=f"step {step}")
+        np.testing.assert_allclose(lr.sigma, ref.sigma, rtol=1e-10, err_msg=f"step {step}")
+        empty = np.setdiff1d(np.arange(dims[j]), idx[:, j])
+        assert not lr.U[j][empty].any()
+        ref.U[j][:] = lr.U[j]
+    np.testing.assert_allclose(als.residual_normsq(), ref.residual_normsq(), rtol=1e-10)
+
+
+def test_exact_then_sampled_update_matches_oracle(ftl, oracle):
+    """The exact-vs-sampled comparison loop of the reference (als.py:16-40):
+    execute_exact_als_update(j) then execute_ds_als_update(j) on one handle."""
+    rng = np.random.default_rng(91)
+    dims, R, J = (40, 30, 25), 5, 600
+    N = len(dims)
+    idx, vals = random_sparse(dims, 4000, rng, unique=True)
+    U0 = [rng.standard_normal((d, R)) for d in dims]
+    ref = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U0], J)
+    lr = ftl.LowRankTensor(R, [u.copy() for u in U0])
+    als = ftl.ALS(lr, ftl.SparseTensor(idx, vals))
+    als.initialize_ds_als(J)
+    for j in range(N):
+        ref.exact_update(j)
+        als.execute_exact_als_update(j, True, True)
+        np.testing.assert_allclose(lr.U[j], ref.U[j], rtol=1e-9, atol=1e-11 * np.abs(ref.U[j]).max())
+        ref.U[j][:] = lr.U[j]
+        draws = rng.random(2 * N * J)
+        if ref.sampler.draw(j, draws)["margins"].min() <= 1e-10:
+            pytest.skip("uniform within 1e-10 of a boundary (seed choice)")
+        d = ref.update(j, draws)
+        als.execute_ds_als_update(j, True, True, random_draws=draws)
+        assert als.last_stats()["found"] == d["found"]
+        np.testing.assert_allclose(lr.U[j], ref.U[j], rtol=1e-9, atol=1e-11 * np.abs(ref.U[j]).max())
+        ref.U[j][:] = lr.U[j]
+        ref.sampler.update_sampler(j)
+
+
+def test_als_prod_exact_is_monotone(ftl):
+    """als_prod(method="exact") (als.py:84-85): the fit never decreases and
+    reaches the noise floor of a fully observed noisy rank-4 tensor."""
+    rng = np.random.default_rng(8)
+    dims, R = (30, 25, 20), 4
+    X = cp_dense([rng.standard_normal((d, R)) for d in dims])
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), axis=-1).reshape(-1, 3)
+    noise = 0.01 * X.std() * rng.standard_normal(X.size)
+    vals = X.reshape(-1) + noise
+    rhs = ftl.PySparseTensor.from_arrays(idx.astype(np.uint32), vals)
+    lhs = ftl.PyLowRank(dims, R, seed=11)
+    lhs.ten.renormalize_columns(-1)
+    rec = ftl.als_prod(lhs, rhs, 0, "exact", 20, 1e-9, epoch_length=1)
+    fits = rec["fits"]
+    assert all(b >= a - 1e-12 for a, b in zip(fits, fits[1:])), fits
+    floor = 1 - np.linalg.norm(noise) / np.linalg.norm(vals)
+    assert fits[-1] > floor - 1e-3, (fits[-1], floor)

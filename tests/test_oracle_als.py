@@ -168,3 +168,94 @@ def test_als_sweeps_raise_fit(oracle):
         for j in range(3):
             als.update(j, rng.random(2 * 3 * J))
     assert als.fit() > 0.95 > f0
+
+
+# ---- exact (unsampled) update: execute_exact_mttkrp / execute_exact_als_update ----
+
+def _dense_mttkrp(X, U, j):
+    """X_(j) times the Khatri-Rao product of the other factors, by einsum."""
+    N = X.ndim
+    letters = "abcdefgh"
+    sub_x = letters[:N]
+    subs = [letters[k] + "r" for k in range(N) if k != j]
+    ops = [U[k] for k in range(N) if k != j]
+    return np.einsum(sub_x + "," + ",".join(subs) + "->" + letters[j] + "r", X, *ops)
+
+
+@pytest.mark.parametrize("dims,nnz,j", [((12, 9, 7), 300, 0), ((12, 9, 7), 300, 2), ((5, 6, 4, 3), 200, 1),
+                                        ((30, 25), 250, 1)])
+def test_exact_mttkrp_matches_dense(oracle, dims, nnz, j):
+    rng = np.random.default_rng(sum(dims) * 7 + j)
+    idx, vals = random_sparse(dims, nnz, rng)
+    R = 6
+    U = [rng.standard_normal((d, R)) for d in dims]
+    S = oracle.OracleSparse(idx, vals)
+    out = S.exact_mttkrp(j, U)
+    ref = _dense_mttkrp(dense_from_sparse(idx, vals, dims), U, j)
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # a row with no nonzeros stays zero
+    empty = np.setdiff1d(np.arange(dims[j]), idx[:, j])
+    assert not out[empty].any()
+
+
+def _dense_exact_solution(X, U, j):
+    """argmin_V ||X_(j) - V KRP^T||_F by least squares on the unfolding."""
+    N = X.ndim
+    other = [k for k in range(N) if k != j]
+    K = np.ones((1, U[0].shape[1]))
+    for k in other:  # rows of the KRP in the C order of the remaining axes
+        K = (K[:, None, :] * U[k][None, :, :]).reshape(-1, U[0].shape[1])
+    Xj = np.moveaxis(X, j, 0).reshape(X.shape[j], -1)
+    sol, *_ = np.linalg.lstsq(K, Xj.T, rcond=None)
+    return sol.T
+
+
+@pytest.mark.parametrize("dims,R,j", [((10, 8, 9), 4, 0), ((10, 8, 9), 4, 2), ((6, 5, 4, 5), 3, 1)])
+def test_exact_update_is_dense_least_squares(oracle, dims, R, j):
+    rng = np.random.default_rng(R * 31 + j)
+    nnz = int(0.6 * np.prod(dims))
+    idx, vals = random_sparse(dims, nnz, rng)
+    U0 = [rng.standard_normal((d, R)) for d in dims]
+    als = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U0], J=16)
+    als.exact_update(j, renormalize=False)
+    ref = _dense_exact_solution(dense_from_sparse(idx, vals, dims), U0, j)
+    np.testing.assert_allclose(als.U[j], ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+    # with renormalize: unit columns, sigma = the norms
+    als2 = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U0], J=16)
+    als2.exact_update(j, renormalize=True)
+    norms = np.linalg.norm(ref, axis=0)
+    np.testing.assert_allclose(als2.sigma, norms, rtol=1e-9)
+    np.testing.assert_allclose(als2.U[j], ref / norms, rtol=1e-9, atol=1e-11)
+    for k in range(len(dims)):
+        if k != j:
+            np.testing.assert_array_equal(als2.U[k], U0[k])
+
+
+def test_exact_update_recovers_lowrank_and_fit_is_monotone(oracle):
+    rng = np.random.default_rng(5)
+    dims, R = (9, 8, 7), 3
+    Ut = [rng.standard_normal((d, R)) for d in dims]
+    X = cp_dense(Ut)
+    idx = np.argwhere(np.ones(dims, dtype=bool)).astype(np.uint32)
+    vals = X[tuple(idx.T)]
+    S = oracle.OracleSparse(idx, vals)
+    U0 = [u.copy() for u in Ut]
+    U0[1] = rng.standard_normal((dims[1], R))
+    als = oracle.OracleALS(S, U0, J=16)
+    als.exact_update(1, renormalize=True)
+    # the fully observed noise-free tensor is reproduced exactly
+    M = cp_dense(als.U, als.sigma)
+    np.testing.assert_allclose(M, X, rtol=1e-9, atol=1e-10 * np.abs(X).max())
+    # noisy data: exact ALS never lowers the fit, and reaches the noise
+    # floor 1 - ||noise|| / ||X + noise|| (the rank-3 fit can only be better)
+    noise = 0.1 * rng.standard_normal(vals.shape)
+    noisy = vals + noise
+    als = oracle.OracleALS(oracle.OracleSparse(idx, noisy), [rng.standard_normal((d, R)) for d in dims], J=16)
+    fits = []
+    for sweep in range(15):
+        for j in range(3):
+            als.exact_update(j, renormalize=True)
+            fits.append(als.fit())
+    assert all(b >= a - 1e-12 for a, b in zip(fits, fits[1:])), fits
+    floor = 1 - np.linalg.norm(noise) / np.linalg.norm(noisy)
+    assert floor < fits[-1] < floor + 0.02
