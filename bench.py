@@ -239,6 +239,20 @@ def als_main(args):
         "sparse_construction_s": t_build,
         "generation_s": t_gen,
     }
+    if args.exact_sweeps:
+        # execute_exact_als_update sweeps continuing from the sampled result;
+        # the first update of each mode also sorts the nonzeros by that mode
+        ex_ms = []
+        for sw in range(args.exact_sweeps):
+            for j in range(N):
+                t = time.perf_counter()
+                als.execute_exact_als_update(j, True, True)
+                ex_ms.append((time.perf_counter() - t) * 1e3)
+            log(f"exact sweep {sw + 1}: updates {', '.join(f'{x:.1f}' for x in ex_ms[-N:])} ms")
+        steady = ex_ms[N:] if len(ex_ms) > N else ex_ms
+        line["exact"] = {"sweeps": args.exact_sweeps, "first_update_per_mode_ms": ex_ms[:N],
+                         "update_ms_mean": float(np.mean(steady)), "s_per_sweep": float(np.mean(steady)) * N / 1e3,
+                         "fit_after": float(fit())}
     if args.als_oracle:
         line.update(als_oracle_compare(host_idx, host_vals, dims, R, J, U_norm, sigma0, draws_all, fits, sweeps))
     print(json.dumps(line), flush=True)
@@ -411,6 +425,8 @@ def main():
     ap.add_argument("--als-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--als-oracle", action="store_true", help="rerun the sweeps on the CPU oracle and compare fits")
     ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--exact-sweeps", type=int, default=0,
+                    help="ALS workload: exact (unsampled) update sweeps after the sampled ones")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="arrival rate of the default run's serving leg")
     ap.add_argument("--serve-rates", default="50,100,200,500,1000,2000")
