@@ -65,6 +65,8 @@ struct Sampler {
     int precision = FTL_FP64;
     uint32_t ts64 = 0, ts32 = 0;
     uint32_t fp32_levels = kDefaultFp32Levels;
+    // FP32 mode: leaf scans over the FP32 factor copy (option "fp32_leaf")
+    uint32_t leaf32 = 1;
     uint64_t tree_pad = 0;  // zeroed entries after every tree (draw kernel over-read)
     std::vector<ModeHost> modes;
     // computeM outputs and scratch.  A sampler that owns its stream keeps two
@@ -307,6 +309,9 @@ struct Sampler {
             FTL_REQUIRE(value == 0 || value == 256 || value == 512 || value == 1024,
                         "jacobi_threads must be 0, 256, 512 or 1024");
             jacobi_override = (uint32_t)value;
+        } else if (key == "fp32_leaf") {
+            FTL_REQUIRE(value == 0 || value == 1, "fp32_leaf must be 0 or 1");
+            leaf32 = (uint32_t)value;
         } else if (key == "fp32_levels") {
             FTL_REQUIRE(value >= 0 && value <= 64, "fp32_levels must be in [0, 64]");
             fp32_levels = value == 0 ? kDefaultFp32Levels : (uint32_t)value;
@@ -327,6 +332,7 @@ struct Sampler {
         if (key == "xcd_buckets") return xcd_buckets;
         if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
+        if (key == "fp32_leaf") return leaf32;
         if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
         throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
     }
@@ -727,6 +733,7 @@ struct Sampler {
         a.ts32 = ts32;
         a.esh = make_shape(R, 1);
         a.pinv = ms.pinv;
+        a.leaf32 = (int)leaf32;
         a.draws = draws;
         a.seed = seed;
         a.call = call;

@@ -287,6 +287,9 @@ struct DrawArgs {
     // between launches in h_state (J x R) when the draw is split by mode
     int mi_begin, mi_end;
     double* h_state;
+    // FP32 samplers: leaf scans over the FP32 factor copy (1) or the FP64
+    // factors (0)
+    int leaf32;
     // subtree-bucketed descent of one mode (k_draw_prefix / k_draw_suffix):
     // per-sample x (J x R) and {low, high, m, node} (J x 4) after the first
     // split_level levels, the level-split_level subtree of each sample and
@@ -648,7 +651,7 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
                 const double frac = (ug - low) / (high - low);
                 const uint64_t r0 = idx * md.sh.F;
                 const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
-                if constexpr (fp32)
+                if (fp32 && a.leaf32)
                     res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
                 else
                     res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g,
