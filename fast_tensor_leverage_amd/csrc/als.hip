@@ -117,6 +117,13 @@ void inclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hi
     FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(p, b, in, out, n, st));
 }
 
+void inclusive_max(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+    size_t b = 0;
+    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b, in, out, hipcub::Max(), n, st));
+    void* p = t.ensure(b);
+    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveScan(p, b, in, out, hipcub::Max(), n, st));
+}
+
 void exclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
     size_t b = 0;
     FTL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
@@ -157,6 +164,7 @@ struct Matches {
     uint32_t *lo = nullptr, *cnt = nullptr, *off = nullptr;
     uint32_t *mrow = nullptr, *srow = nullptr, *flags = nullptr, *segid = nullptr;
     uint32_t *seg_start = nullptr, *seg_row = nullptr, *npieces = nullptr, *piece_off = nullptr;
+    uint32_t *piece_seg = nullptr, *piece_mark = nullptr, *heavy = nullptr;
     uint64_t *mpay = nullptr, *spay = nullptr;
     double* partial = nullptr;
     uint32_t* counters = nullptr;       // device, kCntSlots
@@ -177,7 +185,9 @@ struct Matches {
         if (T <= cap_t && R <= cap_r) return;
         uint64_t c = std::max<uint64_t>(T, cap_t + cap_t / 2);
         for (void* p : {(void*)mrow, (void*)srow, (void*)flags, (void*)segid, (void*)seg_start, (void*)seg_row,
-                        (void*)npieces, (void*)piece_off, (void*)mpay, (void*)spay, (void*)partial})
+                        (void*)npieces, (void*)piece_off, (void*)piece_seg, (void*)piece_mark, (void*)heavy,
+                        (void*)mpay, (void*)spay,
+                        (void*)partial})
             ar.release(p);
         mrow = ar.alloc<uint32_t>(c);
         srow = ar.alloc<uint32_t>(c);
@@ -189,6 +199,9 @@ struct Matches {
         piece_off = ar.alloc<uint32_t>(c);
         mpay = ar.alloc<uint64_t>(c);
         spay = ar.alloc<uint64_t>(c);
+        heavy = ar.alloc<uint32_t>(c);
+        piece_seg = ar.alloc<uint32_t>(c + c / kSegPiece + 1);
+        piece_mark = ar.alloc<uint32_t>(c + c / kSegPiece + 1);
         // pieces W <= D + T / kSegPiece <= T + T / kSegPiece + 1
         cap_r = std::max<uint64_t>(R, cap_r);
         partial = ar.alloc<double>((c + c / kSegPiece + 1) * cap_r);
@@ -229,6 +242,7 @@ struct SparseDev {
     // built on first use
     struct RowOrder {
         uint32_t *pos = nullptr, *seg_start = nullptr, *seg_row = nullptr, *piece_off = nullptr;
+        uint32_t *piece_seg = nullptr, *heavy = nullptr;  // segment of each piece; rows of > kHeavyPieces pieces
         uint32_t* counters = nullptr;  // kCntSlots, kCntRows = D, kCntPieces = W
         uint64_t D = 0, W = 0;
     };
@@ -374,6 +388,17 @@ struct SparseDev {
                                      stream));
         FTL_HIP_CHECK(hipMemcpyAsync(o.seg_row, srows, o.D * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
         FTL_HIP_CHECK(hipMemcpyAsync(o.piece_off, poff, o.D * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        // segment of every piece (max-scan of the segment starts) and the heavy rows
+        o.piece_seg = mem.alloc<uint32_t>(o.W);
+        o.heavy = mem.alloc<uint32_t>(o.D);
+        FTL_HIP_CHECK(hipMemsetAsync(flags, 0, o.W * sizeof(uint32_t), stream));
+        hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(o.D, 256, 8192)), dim3(256), 0, stream, o.piece_off, o.counters,
+                           flags);
+        FTL_LAUNCH_CHECK();
+        inclusive_max(cub, flags, o.piece_seg, o.W, stream);
+        hipLaunchKernelGGL(k_heavy_list, dim3(grid_for(o.D, 256, 2048)), dim3(256), 0, stream, o.piece_off, o.counters,
+                           o.heavy);
+        FTL_LAUNCH_CHECK();
         return o;
         // stage's buffers are freed after the stream drained
     }
@@ -410,14 +435,26 @@ void probe_and_count(SparseDev& X, uint32_t j, const uint32_t* tup, const uint32
 
 // Returns the block count of the row-finalise kernel (the number of column
 // partials written to colsq when it is given).
+// Rows of more than kHeavyPieces pieces get their pieces summed by a block
+// each (into the row's first piece) before the row finalise.
+void reduce_heavy(const uint32_t* piece_off, uint32_t* counters, uint32_t* list, uint64_t D_cap, uint32_t R,
+                  double* partial, hipStream_t st) {
+    FTL_HIP_CHECK(hipMemsetAsync(counters + kCntHeavy, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_heavy_list, dim3(grid_for(D_cap, 256, 2048)), dim3(256), 0, st, piece_off, counters, list);
+    FTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_heavy_sum, dim3(512), dim3(256), 0, st, piece_off, counters, list, R, partial);
+    FTL_LAUNCH_CHECK();
+}
+
 template <int G, int CPL>
 unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
                        const double* pinv, double* out, double* colsq, hipStream_t st) {
     constexpr int GPB = 256 / G;
     const unsigned blocks = grid_for(T, GPB, 16384);
     hipLaunchKernelGGL((k_item_partial<G, CPL>), dim3(blocks), dim3(256), 0, st, m.seg_start, m.piece_off,
-                       m.counters, m.spay, X.val[j], lhs, R, m.partial);
+                       m.piece_seg, m.counters, m.spay, X.val[j], lhs, R, m.partial);
     FTL_LAUNCH_CHECK();
+    reduce_heavy(m.piece_off, m.counters, m.heavy, std::min<uint64_t>(T, X.dims[j]), R, m.partial, st);
     const unsigned fblocks = grid_for(T, GPB, kSegBlocksMax);
     if (pinv)
         hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(fblocks), dim3(256), 0, st, m.seg_row, m.piece_off,
@@ -453,6 +490,14 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, T, m.counters,
                        (int)kCntPieces);
     FTL_LAUNCH_CHECK();
+    // segment of every piece (W <= T: a segment of n >= 1 matches has
+    // ceil(n / kSegPiece) <= n pieces)
+    const uint64_t wcap = T;
+    FTL_HIP_CHECK(hipMemsetAsync(m.piece_mark, 0, wcap * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(T, 256, 8192)), dim3(256), 0, st, m.piece_off, m.counters,
+                       m.piece_mark);
+    FTL_LAUNCH_CHECK();
+    inclusive_max(cub, m.piece_mark, m.piece_seg, wcap, st);
     if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
     if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
     return launch_reduce<64, 2>(X, j, m, T, lhs, R, pinv, out, colsq, st);
@@ -688,7 +733,10 @@ struct Als {
         FactorPtrs F{};
         for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
         hipLaunchKernelGGL((k_exact_item_partial<G, CPL>), dim3(grid_for(o.W, GPB, 16384)), dim3(256), 0, stream,
-                           o.seg_start, o.piece_off, o.counters, o.pos, X->keys[j], X->val[j], X->layout[j], F,
+                           o.seg_start, o.piece_off, o.piece_seg, o.counters, o.pos, X->keys[j], X->val[j],
+                           X->layout[j], F, (uint32_t)R, exact_partial);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_seg_heavy_sum, dim3(512), dim3(256), 0, stream, o.piece_off, o.counters, o.heavy,
                            (uint32_t)R, exact_partial);
         FTL_LAUNCH_CHECK();
         const unsigned fblocks = grid_for(o.D, GPB, kSegBlocksMax);

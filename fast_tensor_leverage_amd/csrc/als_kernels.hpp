@@ -36,6 +36,9 @@ namespace ftl {
 
 // Matches of one output row summed sequentially per piece.
 constexpr uint32_t kSegPiece = 256;
+// A row with more pieces than this has them summed by a whole block
+// (k_seg_heavy_sum) before the row finalise.
+constexpr uint32_t kHeavyPieces = 32;
 // Blocks of the partial-sum kernels (fixed per launch shape, so the
 // association of every sum is fixed).
 constexpr uint32_t kGramBlocks = 128;      // sampled Gram (J' rows)
@@ -61,8 +64,9 @@ struct FactorPtrs {
 };
 
 // Counters the ALS pipeline keeps on the device: unique samples J', matched
-// nonzeros T, touched rows D, reduction pieces W.
-enum { kCntUnique = 0, kCntMatches = 1, kCntRows = 2, kCntPieces = 3, kCntSlots = 4 };
+// nonzeros T, touched rows D, reduction pieces W, heavy rows (segments of more
+// than kHeavyPieces pieces).
+enum { kCntUnique = 0, kCntMatches = 1, kCntRows = 2, kCntPieces = 3, kCntHeavy = 4, kCntSlots = 5 };
 
 __device__ inline uint64_t pack_tuple(const KeyLayout& L, const uint32_t* t) {
     uint64_t key = 0;
@@ -436,22 +440,18 @@ __global__ void k_seg_pieces(const uint32_t* __restrict__ seg_start, const uint3
 template <int G, int CPL>
 __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict__ seg_start,
                                                       const uint32_t* __restrict__ piece_off,
+                                                      const uint32_t* __restrict__ piece_seg,
                                                       const uint32_t* __restrict__ counters,
                                                       const uint64_t* __restrict__ spay,
                                                       const double* __restrict__ valj,
                                                       const double* __restrict__ lhs, uint32_t R,
                                                       double* __restrict__ partial) {
 #pragma clang fp contract(off)
-    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t W = counters[kCntPieces];
     const uint32_t lane = threadIdx.x % G;
     const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
     for (uint64_t w = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; w < W; w += groups) {
-        uint32_t l = 0, h = D;  // segment of piece w: largest d with piece_off[d] <= w
-        while (h - l > 1) {
-            const uint32_t mid = (l + h) >> 1;
-            if (piece_off[mid] <= w) l = mid;
-            else h = mid;
-        }
+        const uint32_t l = piece_seg[w];  // segment of piece w
         const uint64_t i0 = (uint64_t)seg_start[l] + (uint64_t)(w - piece_off[l]) * kSegPiece;
         const uint64_t i1 = min<uint64_t>(i0 + kSegPiece, seg_start[l + 1]);
         double acc[CPL];
@@ -494,22 +494,18 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
 template <int G, int CPL>
 __global__ void __launch_bounds__(256) k_exact_item_partial(const uint32_t* __restrict__ seg_start,
                                                             const uint32_t* __restrict__ piece_off,
+                                                            const uint32_t* __restrict__ piece_seg,
                                                             const uint32_t* __restrict__ counters,
                                                             const uint32_t* __restrict__ pos,
                                                             const uint64_t* __restrict__ keys,
                                                             const double* __restrict__ vals, KeyLayout L,
                                                             FactorPtrs F, uint32_t R, double* __restrict__ partial) {
 #pragma clang fp contract(off)
-    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t W = counters[kCntPieces];
     const uint32_t lane = threadIdx.x % G;
     const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
     for (uint64_t w = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; w < W; w += groups) {
-        uint32_t l = 0, h = D;  // segment of piece w: largest d with piece_off[d] <= w
-        while (h - l > 1) {
-            const uint32_t mid = (l + h) >> 1;
-            if (piece_off[mid] <= w) l = mid;
-            else h = mid;
-        }
+        const uint32_t l = piece_seg[w];  // segment of piece w
         const uint64_t i0 = (uint64_t)seg_start[l] + (uint64_t)(w - piece_off[l]) * kSegPiece;
         const uint64_t i1 = min<uint64_t>(i0 + kSegPiece, seg_start[l + 1]);
         double acc[CPL];
@@ -566,6 +562,59 @@ __global__ void k_iota(uint32_t* __restrict__ out, uint64_t n) {
         out[i] = (uint32_t)i;
 }
 
+// piece_seg[piece_off[d]] = d for every segment (the entries between stay
+// zero); an inclusive max-scan then gives every piece its segment.
+__global__ void k_mark_segs(const uint32_t* __restrict__ piece_off, const uint32_t* __restrict__ counters,
+                            uint32_t* __restrict__ piece_seg) {
+    const uint64_t D = counters[kCntRows];
+    for (uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; d < D; d += (uint64_t)gridDim.x * blockDim.x)
+        piece_seg[piece_off[d]] = (uint32_t)d;
+}
+
+// Rows with more than kHeavyPieces pieces, listed in any order (each is
+// reduced on its own, so the order does not change any result).
+__global__ void k_heavy_list(const uint32_t* __restrict__ piece_off, uint32_t* __restrict__ counters,
+                             uint32_t* __restrict__ list) {
+    const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
+    for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < D; d += gridDim.x * blockDim.x) {
+        const uint32_t p1 = d + 1 < D ? piece_off[d + 1] : W;
+        if (p1 - piece_off[d] > kHeavyPieces) list[atomicAdd(&counters[kCntHeavy], 1u)] = d;
+    }
+}
+
+// Sum of the pieces of every heavy row into its first piece, with a fixed
+// association: 8 piece lanes per column sum every 8th piece in order, then
+// the 8 lane sums are added in lane order.  One block per row at a time.
+__global__ void __launch_bounds__(256) k_seg_heavy_sum(const uint32_t* __restrict__ piece_off,
+                                                       const uint32_t* __restrict__ counters,
+                                                       const uint32_t* __restrict__ list, uint32_t R,
+                                                       double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double red[8][32];
+    const uint32_t nh = counters[kCntHeavy], D = counters[kCntRows], W = counters[kCntPieces];
+    const uint32_t l = threadIdx.x / 32, c = threadIdx.x % 32;
+    for (uint32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+        const uint32_t d = list[i];
+        const uint64_t p0 = piece_off[d], p1 = d + 1 < D ? piece_off[d + 1] : W;
+        for (uint32_t c0 = 0; c0 < R; c0 += 32) {
+            const uint32_t col = c0 + c;
+            double s = 0.0;
+            if (col < R) {
+#pragma unroll 4
+                for (uint64_t p = p0 + l; p < p1; p += 8) s = s + partial[p * R + col];
+            }
+            red[l][c] = s;
+            __syncthreads();
+            if (l == 0 && col < R) {
+                double t = red[0][c];
+                for (int q = 1; q < 8; q++) t = t + red[q][c];
+                partial[p0 * R + col] = t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Segment d: sum of its pieces in order, then either the row itself (plain
 // MTTKRP) or its product with pinv (the ALS solve U_j = MTTKRP pinv,
 // als.hpp:189-203), written to row seg_row[d] of out.  With colsq given, the
@@ -605,7 +654,9 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
         const bool valid = d < D;
         double mt[CPL];
         if (valid) {
-            const uint32_t p0 = piece_off[d], p1 = d + 1 < D ? piece_off[d + 1] : W;
+            const uint32_t p0 = piece_off[d];
+            uint32_t p1 = d + 1 < D ? piece_off[d + 1] : W;
+            if (p1 - p0 > kHeavyPieces) p1 = p0 + 1;  // k_seg_heavy_sum left the row's sum in piece p0
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 const uint32_t c = lane + q * G;
