@@ -647,58 +647,49 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
     for (int q = 0; q < CPL; q++) sq[q] = 0.0;
     __syncthreads();
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
-    // every group runs the same number of iterations (block-uniform barrier)
-    const uint64_t iters = (D + groups - 1) / groups;
-    for (uint64_t it = 0; it < iters; it++) {
-        const uint64_t d = it * groups + (uint64_t)blockIdx.x * GPB + gi;
-        const bool valid = d < D;
+    // groups run independently: a group stages its row in its own LDS slot,
+    // and a group is at most one wave, whose LDS accesses are ordered
+    for (uint64_t d = (uint64_t)blockIdx.x * GPB + gi; d < D; d += groups) {
         double mt[CPL];
-        if (valid) {
-            const uint32_t p0 = piece_off[d];
-            uint32_t p1 = d + 1 < D ? piece_off[d + 1] : W;
-            if (p1 - p0 > kHeavyPieces) p1 = p0 + 1;  // k_seg_heavy_sum left the row's sum in piece p0
+        const uint32_t p0 = piece_off[d];
+        uint32_t p1 = d + 1 < D ? piece_off[d + 1] : W;
+        if (p1 - p0 > kHeavyPieces) p1 = p0 + 1;  // k_seg_heavy_sum left the row's sum in piece p0
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const uint32_t c = lane + q * G;
+            double s = 0.0;
+            if (c < R) {
+                s = partial[(uint64_t)p0 * R + c];
+                for (uint32_t p = p0 + 1; p < p1; p++) s = s + partial[(uint64_t)p * R + c];
+            }
+            mt[q] = s;
+        }
+        if constexpr (!SOLVE) {
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 const uint32_t c = lane + q * G;
-                double s = 0.0;
-                if (c < R) {
-                    s = partial[(uint64_t)p0 * R + c];
-                    for (uint32_t p = p0 + 1; p < p1; p++) s = s + partial[(uint64_t)p * R + c];
-                }
-                mt[q] = s;
-            }
-        }
-        if constexpr (!SOLVE) {
-            if (valid) {
-#pragma unroll
-                for (int q = 0; q < CPL; q++) {
-                    const uint32_t c = lane + q * G;
-                    if (c < R) out[(uint64_t)seg_row[d] * R + c] = mt[q];
-                }
+                if (c < R) out[(uint64_t)seg_row[d] * R + c] = mt[q];
             }
         } else {
-            if (valid) {
 #pragma unroll
-                for (int q = 0; q < CPL; q++) mt_s[gi][lane + q * G] = mt[q];
-            }
-            __syncthreads();
-            if (valid) {
-                const uint64_t row = seg_row[d];
+            for (int q = 0; q < CPL; q++) mt_s[gi][lane + q * G] = mt[q];
+            wave_sync_lds();
+            const uint64_t row = seg_row[d];
 #pragma unroll
-                for (int q = 0; q < CPL; q++) {
-                    const uint32_t c = lane + q * G;
-                    if (c < R) {
-                        double s = 0.0;
-                        for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * P[k * R + c];
-                        out[row * R + c] = s;
-                        sq[q] = sq[q] + s * s;
-                    }
+            for (int q = 0; q < CPL; q++) {
+                const uint32_t c = lane + q * G;
+                if (c < R) {
+                    double s = 0.0;
+                    for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * P[k * R + c];
+                    out[row * R + c] = s;
+                    sq[q] = sq[q] + s * s;
                 }
             }
-            __syncthreads();
+            wave_sync_lds();  // this row's reads of mt_s before the next row's writes
         }
     }
     if (SOLVE && colsq) {
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < CPL; q++) sq_s[gi][lane + q * G] = sq[q];
         __syncthreads();
