@@ -234,28 +234,47 @@ struct Sampler {
         ba.R = (uint32_t)R;
         ba.ts64 = ts64;
         ba.first_leaf = m.sh.node_count - m.sh.leaf_count;
-        const size_t lds = (size_t)m.F * R * sizeof(double);
-        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)m.sh.leaf_count), dim3(256), lds, st, ba);
+        // leaves per block: their row blocks within 32 KB of LDS (a power of
+        // two <= 8, at least one leaf)
+        const uint64_t leaf_doubles = std::max<uint64_t>(1, m.F * R);
+        uint32_t lpb = 1;
+        while (lpb < 8 && 2 * lpb * leaf_doubles <= 4096) lpb *= 2;
+        const size_t lds = (size_t)lpb * leaf_doubles * sizeof(double) + tri_size(R) * sizeof(uint16_t);
+        const uint64_t lblocks = (m.sh.leaf_count + lpb - 1) / lpb;
+        FTL_REQUIRE(lblocks <= 0x7fffffffull, "too many leaves for one launch");
+        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)lblocks), dim3(256), lds, st, ba, lpb);
         FTL_HIP_CHECK(hipGetLastError());
-        int64_t start = (int64_t)m.sh.nodes_before_lfill, end = (int64_t)ba.first_leaf;
-        for (int64_t level = (int64_t)m.sh.lfill_level; level >= 0; level--) {
-            if (end > start) {
-                const uint64_t work = (uint64_t)(end - start) * ts64;
-                hipLaunchKernelGGL(k_level_sum, dim3(grid_for(work, 256)), dim3(256), 0, st, m.tree64,
-                                   (uint64_t)start, (uint64_t)end, ts64);
-                FTL_HIP_CHECK(hipGetLastError());
+        auto grid_nodes = [](uint64_t n) { return dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(n, 1), 1u << 20)); };
+        // internal nodes of the lfill level (parents of a partial bottom level)
+        const uint64_t start = m.sh.nodes_before_lfill, end = ba.first_leaf;
+        if (end > start) {
+            hipLaunchKernelGGL(k_level_sum, grid_nodes(end - start), dim3(256), 0, st, m.tree64, start, end - start,
+                               ts64);
+            FTL_HIP_CHECK(hipGetLastError());
+        }
+        // levels lfill_level - 1 .. 0 are complete: three per launch
+        for (int64_t lv = (int64_t)m.sh.lfill_level - 1; lv >= 0;) {
+            if (lv >= 2) {
+                const uint64_t count = 1ull << (lv - 2);
+                hipLaunchKernelGGL(k_level_sum3, grid_nodes(count), dim3(256), 0, st, m.tree64, count - 1, count,
+                                   ts64);
+                lv -= 3;
+            } else {
+                const uint64_t count = 1ull << lv;
+                hipLaunchKernelGGL(k_level_sum, grid_nodes(count), dim3(256), 0, st, m.tree64, count - 1, count,
+                                   ts64);
+                lv -= 1;
             }
-            end = start;
-            start = ((start + 1) / 2) - 1;
+            FTL_HIP_CHECK(hipGetLastError());
         }
         if (precision == FTL_FP32) {
             const uint64_t nel = m.I * R;
             hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, st, m.U64, m.U32, nel);
             FTL_HIP_CHECK(hipGetLastError());
             if (m.tree32) {
-                const uint64_t work = (m.sh.node_count - m.n64) * ts32;
-                hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(work, 256)), dim3(256), 0, st, m.tree64,
-                                   m.tree32, m.n64, m.sh.node_count, ts64, ts32);
+                const uint64_t nodes = m.sh.node_count - m.n64;
+                hipLaunchKernelGGL(k_tree_to_f32, grid_nodes(nodes), dim3(256), 0, st, m.tree64, m.tree32, m.n64,
+                                   nodes, ts64, ts32);
                 FTL_HIP_CHECK(hipGetLastError());
             }
         }
@@ -275,8 +294,9 @@ struct Sampler {
         m.tree32_elems = n32 + tree_pad;
         m.tree32 = dalloc<float>(m.tree32_elems);
         FTL_HIP_CHECK(hipMemsetAsync(m.tree32 + n32, 0, tree_pad * sizeof(float), stream));
-        hipLaunchKernelGGL(k_tree_to_f32, dim3(grid_for(n32, 256)), dim3(256), 0, stream, m.tree64, m.tree32, m.n64,
-                           m.sh.node_count, ts64, ts32);
+        const uint64_t nodes = m.sh.node_count - m.n64;
+        hipLaunchKernelGGL(k_tree_to_f32, dim3((unsigned)std::min<uint64_t>(nodes, 1u << 20)), dim3(256), 0, stream,
+                           m.tree64, m.tree32, m.n64, nodes, ts64, ts32);
         FTL_HIP_CHECK(hipGetLastError());
     }
 

@@ -26,10 +26,11 @@
 namespace ftl {
 
 // ---------------------------------------------------------------------------
-// Tree build kernels (FP64).  One block per leaf: the leaf's row block is
-// staged in LDS, each thread produces packed upper-triangle entries as
-// sequential sums over the rows (no contraction, so the result is the exactly
-// rounded sequential sum the reference's dsyrk restatement produces).
+// Tree build kernels (FP64).  Leaf blocks are staged in LDS and each packed
+// upper-triangle entry is a sequential sum over the leaf's rows (no
+// contraction, so the result is the exactly rounded sequential sum the
+// reference's dsyrk restatement produces); every internal node is the sum of
+// its two children.
 // ---------------------------------------------------------------------------
 struct BuildArgs {
     const double* U;   // n x R
@@ -40,53 +41,92 @@ struct BuildArgs {
     uint64_t first_leaf;  // node_count - leaf_count
 };
 
+// lpb leaves per block (BS / lpb threads each).  Dynamic LDS: the leaves'
+// row blocks (lpb x F x R doubles), then the (row, column) of every packed
+// entry (T uint16).  Entry e of a leaf is the sequential sum over its rows.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a) {
+__global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a, uint32_t lpb) {
 #pragma clang fp contract(off)
     extern __shared__ double lds_u[];
-    const uint64_t i = blockIdx.x;  // leaf position
-    const uint64_t node = a.first_leaf + i;
-    const uint64_t idx = leaf_index(a.sh, node);
     const uint32_t R = a.R;
-    const uint64_t r0 = idx * a.sh.F;
-    const uint32_t rows = (uint32_t)min<uint64_t>(a.sh.F, a.sh.n - r0);
-    for (uint32_t e = threadIdx.x; e < rows * R; e += BS) lds_u[e] = a.U[r0 * R + e];
-    __syncthreads();
+    const uint32_t F = (uint32_t)a.sh.F;
     const uint32_t T = (uint32_t)tri_size(R);
+    uint16_t* pairs = reinterpret_cast<uint16_t*>(lds_u + (uint64_t)lpb * F * R);
+    for (uint32_t r = threadIdx.x; r < R; r += BS) {
+        const uint32_t base = r * R - r * (r - 1) / 2;
+        for (uint32_t c = r; c < R; c++) pairs[base + c - r] = (uint16_t)(r | (c << 8));
+    }
+    const uint32_t tpl = BS / lpb;
+    const uint32_t q = threadIdx.x / tpl, lt = threadIdx.x - q * tpl;
+    const uint64_t i = (uint64_t)blockIdx.x * lpb + q;  // leaf position
+    const bool have = q < lpb && i < a.sh.leaf_count;
+    uint64_t node = 0, r0 = 0;
+    uint32_t rows = 0;
+    double* lu = lds_u + (uint64_t)q * F * R;
+    if (have) {
+        node = a.first_leaf + i;
+        r0 = leaf_index(a.sh, node) * F;
+        rows = (uint32_t)min<uint64_t>(F, a.sh.n - r0);
+        const double* src = a.U + r0 * R;
+        for (uint32_t e = lt; e < rows * R; e += tpl) lu[e] = src[e];
+    }
+    __syncthreads();
+    if (!have) return;
     double* out = a.tree + node * (uint64_t)a.ts64;
-    for (uint32_t e = threadIdx.x; e < a.ts64; e += BS) {
+    for (uint32_t e = lt; e < a.ts64; e += tpl) {
         double acc = 0.0;
         if (e < T) {
-            // invert e -> (row a, col b)
-            uint32_t ra = 0, off = e;
-            while (off >= R - ra) {
-                off -= R - ra;
-                ra++;
-            }
-            const uint32_t cb = ra + off;
-            for (uint32_t t = 0; t < rows; t++)
-                acc = acc + lds_u[t * R + ra] * lds_u[t * R + cb];
+            const uint32_t ab = pairs[e];
+            const uint32_t ra = ab & 0xffu, cb = ab >> 8;
+            for (uint32_t t = 0; t < rows; t++) acc = acc + lu[t * R + ra] * lu[t * R + cb];
         }
         out[e] = acc;
     }
 }
 
-__global__ void k_level_sum(double* tree, uint64_t start, uint64_t end, uint32_t ts64) {
-    const uint64_t total = (end - start) * ts64;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t c = start + w / ts64, e = w % ts64;
-        tree[c * ts64 + e] = tree[(2 * c + 1) * ts64 + e] + tree[(2 * c + 2) * ts64 + e];
+// Nodes [start, start + count) of one level: parent = left + right, a block
+// per node.
+__global__ void k_level_sum(double* tree, uint64_t start, uint64_t count, uint32_t ts64) {
+    for (uint64_t b = blockIdx.x; b < count; b += gridDim.x) {
+        const uint64_t c = start + b;
+        double* p = tree + c * ts64;
+        const double* l = tree + (2 * c + 1) * ts64;
+        const double* r = tree + (2 * c + 2) * ts64;
+        for (uint32_t e = threadIdx.x; e < ts64; e += blockDim.x) p[e] = l[e] + r[e];
     }
 }
 
-__global__ void k_tree_to_f32(const double* tree64, float* tree32, uint64_t n64, uint64_t node_count,
-                              uint32_t ts64, uint32_t ts32) {
-    const uint64_t total = (node_count - n64) * ts32;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t c = n64 + w / ts32, e = w % ts32;
-        tree32[w] = e < ts64 ? (float)tree64[c * ts64 + e] : 0.0f;
+// Three complete levels at once: a block takes the subtree of root
+// c = root0 + b (b < count), reads its 8 great-grandchildren and writes the 4
+// grandchildren, 2 children and the root, each the sum of its two children
+// (the same additions as three single-level passes).
+__global__ void k_level_sum3(double* tree, uint64_t root0, uint64_t count, uint32_t ts64) {
+    for (uint64_t b = blockIdx.x; b < count; b += gridDim.x)
+    for (uint32_t e = threadIdx.x; e < ts64; e += blockDim.x) {
+        const uint64_t c = root0 + b;
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = tree[(8 * c + 7 + k) * ts64 + e];
+        double g[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            g[k] = v[2 * k] + v[2 * k + 1];
+            tree[(4 * c + 3 + k) * ts64 + e] = g[k];
+        }
+        const double c0 = g[0] + g[1], c1 = g[2] + g[3];
+        tree[(2 * c + 1) * ts64 + e] = c0;
+        tree[(2 * c + 2) * ts64 + e] = c1;
+        tree[c * ts64 + e] = c0 + c1;
+    }
+}
+
+// FP32 copy of nodes [n64, n64 + count): a block per node.
+__global__ void k_tree_to_f32(const double* tree64, float* tree32, uint64_t n64, uint64_t count, uint32_t ts64,
+                              uint32_t ts32) {
+    for (uint64_t b = blockIdx.x; b < count; b += gridDim.x) {
+        const double* src = tree64 + (n64 + b) * ts64;
+        float* dst = tree32 + b * ts32;
+        for (uint32_t e = threadIdx.x; e < ts32; e += blockDim.x) dst[e] = e < ts64 ? (float)src[e] : 0.0f;
     }
 }
 
