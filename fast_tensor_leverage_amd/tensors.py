@@ -65,6 +65,22 @@ class PySparseTensor:
         self._init(idx, values, lookup, preprocessing, dims=[int(d) for d in max_idxs])
 
     @classmethod
+    def from_npz(cls, filename, lookup="sort", preprocessing=None):
+        """The datasets of the reference's HDF5 tensor files (MODE_0 ..
+        MODE_{N-1}, VALUES, MIN_MODE_SET, MAX_MODE_SET, tensors.py:85-100) stored
+        in a NumPy .npz (e.g. ``numpy.savez(out, **{k: h5[k][:] for k in h5})``
+        where h5py is available)."""
+        idx, values, dims = read_npz(filename)
+        return cls.from_arrays(idx, values, lookup, preprocessing, dims)
+
+    @classmethod
+    def from_tns(cls, filename, lookup="sort", preprocessing=None, one_based=True):
+        """A FROSTT .tns text file: one nonzero per line, N coordinates then
+        the value, whitespace separated, 1-based coordinates ('#' comments)."""
+        idx, values, dims = read_tns(filename, one_based)
+        return cls.from_arrays(idx, values, lookup, preprocessing, dims)
+
+    @classmethod
     def from_arrays(cls, idx, values, lookup="sort", preprocessing=None, dims=None):
         obj = cls.__new__(cls)
         obj._init(np.ascontiguousarray(idx, dtype=np.uint32), np.ascontiguousarray(values, dtype=np.float64),
@@ -87,3 +103,36 @@ class PySparseTensor:
         self.min_idxs = np.zeros(self.N, dtype=np.int64)
         self.ten = SparseTensor(idx, values, lookup, dims=[max(int(d), int(idx[:, k].max()) + 1)
                                                            for k, d in enumerate(self.dims)])
+
+
+def read_npz(filename):
+    """(idx nnz x N uint32, values float64, dims) from an .npz with the
+    reference's HDF5 dataset names: coordinates minus MIN_MODE_SET, dims =
+    MAX_MODE_SET (tensors.py:88-100)."""
+    with np.load(filename, allow_pickle=False) as f:
+        max_idxs = np.asarray(f["MAX_MODE_SET"])
+        min_idxs = np.asarray(f["MIN_MODE_SET"])
+        N = len(max_idxs)
+        nnz = len(f["MODE_0"])
+        idx = np.zeros((nnz, N), dtype=np.uint32)
+        for i in range(N):
+            idx[:, i] = np.asarray(f[f"MODE_{i}"]) - min_idxs[i]
+        values = np.ascontiguousarray(f["VALUES"], dtype=np.float64)
+    return idx, values, [int(d) for d in max_idxs]
+
+
+def read_tns(filename, one_based=True):
+    """(idx nnz x N uint32, values float64, dims) from FROSTT .tns text; dims =
+    the largest coordinate + 1 of each mode."""
+    import pandas as pd
+    df = pd.read_csv(filename, sep=r"\s+", header=None, comment="#", engine="c", float_precision="round_trip")
+    a = df.to_numpy()
+    if a.ndim != 2 or a.shape[1] < 2:
+        raise ValueError(f"{filename}: expected N coordinates and a value per line")
+    coords = a[:, :-1].astype(np.int64) - (1 if one_based else 0)
+    if coords.size and coords.min() < 0:
+        raise ValueError(f"{filename}: coordinate below {1 if one_based else 0}")
+    idx = np.ascontiguousarray(coords, dtype=np.uint32)
+    values = np.ascontiguousarray(a[:, -1], dtype=np.float64)
+    dims = [int(idx[:, k].max()) + 1 if idx.shape[0] else 1 for k in range(idx.shape[1])]
+    return idx, values, dims
