@@ -405,6 +405,128 @@ def c3_main(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Larsen-Kolda sampler (--workload lk)
+# ---------------------------------------------------------------------------
+def lk_update_bytes(I, R):
+    """Algorithmic HBM bytes of update_sampler(k) on an I x R FP64 factor: U
+    read twice (its Gram, then the leverage scores, which need the pinv of
+    the whole Gram), scores written and read once by the prefix pass, prefix
+    sums written."""
+    return 2 * I * R * 8 + 3 * I * 8
+
+
+def lk_main(args):
+    """Sampler(U, J, R, "larsen_kolda") at N=3, I=2^23, R=32, J=50,000, all
+    modes sampled (j = N+1), with h and weights: draw samples/s (factors and
+    uniforms resident in HBM) and update_sampler(k) on the resident factor,
+    both timed with HIP events on the stream the kernels run on."""
+    import torch
+    sys.path.insert(0, HERE)
+    import fast_tensor_leverage_amd as ftl
+    N, I, R, J = N_MODES, args.I, RANK, J_SAMPLES
+    j = N + 1
+    log(f"LK: generating factors N={N} I={I} R={R}")
+    U = [np.random.default_rng(1000 + k).standard_normal((I, R)) for k in range(N)]
+    draws = np.random.default_rng(7).random(2 * N * J)
+    dev = torch.device("cuda", 0)
+    d_draws = torch.from_numpy(draws).to(dev)
+    t0 = time.perf_counter()
+    s = ftl.Sampler(U, J, R, "larsen_kolda")
+    t_build = time.perf_counter() - t0
+    d_s = torch.zeros((N, J), dtype=torch.int64, device=dev)
+    d_h = torch.empty((J, R), dtype=torch.float64, device=dev)
+    d_w = torch.empty((J,), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        s.draw_device(j, d_s, d_h, d_w, d_draws, st)
+        for k in range(N):
+            s.update_sampler(k)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(args.steps):
+        s.draw_device(j, d_s, d_h, d_w, d_draws, st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    draw_el = time.perf_counter() - t0
+    draw_ms = e0.elapsed_time(e1) / args.steps
+    # update_sampler(k) on the resident factor, on the sampler's own stream
+    ext = torch.cuda.ExternalStream(s.stream(), device=dev)
+    u0, u1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    u0.record(ext)
+    for _ in range(args.steps):
+        for k in range(N):
+            s.update_sampler(k)
+    u1.record(ext)
+    torch.cuda.synchronize()
+    upd_ms = u0.elapsed_time(u1) / (args.steps * N)
+    ubytes = lk_update_bytes(I, R)
+    achieved = ubytes / (upd_ms * 1e-3)
+    gpu_samples = d_s.cpu().numpy().view(np.uint64).copy()
+    gpu_w = d_w.cpu().numpy().copy()
+    log(f"LK: draw {draw_ms * 1e3:.1f} us ({J / (draw_ms * 1e-3):,.0f} samples/s); update_sampler "
+        f"{upd_ms:.3f} ms per mode ({achieved / 1e9:.0f} GB/s)")
+    cpu = None
+    cmp = {}
+    if not args.skip_cpu:
+        # oracle on the first 2^20 rows of each factor (its update scales
+        # linearly in I): construction, one update_sampler, the draw of all J
+        # samples; the index check runs the GPU sampler on the same rows
+        from oracle import oracle as orc
+        orc.build()
+        Is = min(I, 2 ** 20)
+        Us = [u[:Is].copy() for u in U]
+        t0 = time.perf_counter()
+        ref = orc.OracleLKSampler(Us, J, R)
+        t_create = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.update_sampler(0)
+        t_upd = time.perf_counter() - t0
+        ref.draw(j, draws)
+        t0 = time.perf_counter()
+        out = ref.draw(j, draws)
+        t_draw = time.perf_counter() - t0
+        small = ftl.Sampler(Us, J, R, "larsen_kolda")
+        gs = np.zeros((N, J), dtype=np.uint64)
+        gh = np.zeros((J, R))
+        gw = np.zeros(J)
+        small.KRPDrawSamples_materialize(j, gs, gh, gw, draws)
+        sure = out["margins"] > 1e-9
+        diff = np.any(gs != out["samples"], axis=0)
+        cpu = {"value": J / t_draw, "unit": "samples/s", "cores": 1, "kind": "port",
+               "sample": f"oracle LarsenKoldaSampler on the first {Is} rows of each factor: draw of J={J} "
+                         f"(all modes, h and weights) {t_draw * 1e3:.1f} ms; update_sampler(0) {t_upd:.2f} s "
+                         f"(x{I / Is:.0f} for I={I}: {t_upd * I / Is:.1f} s); construction {t_create:.1f} s; "
+                         f"host {lscpu_model()}",
+               "update_s_scaled_to_I": t_upd * I / Is}
+        cmp = {"oracle_vs_gpu_mismatches": int(diff.sum()),
+               "oracle_vs_gpu_mismatches_outside_margin": int((diff & sure).sum()),
+               "oracle_compared_samples": J,
+               "oracle_weights_max_rel_diff": float(np.max(np.abs(gw[~diff] - out["weights"][~diff])
+                                                           / out["weights"][~diff]))}
+        log(f"LK CPU oracle: draw {t_draw * 1e3:.1f} ms, update {t_upd:.2f} s at I={Is}; {cmp}")
+        del small
+    line = {"metric": "Larsen-Kolda KRP samples/s (N=3,I=2^23,R=32), update_sampler ms",
+            "value": J / (draw_ms * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * draw_el / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic: N(0,1) factors rng(1000+k), uniforms rng(7), resident in HBM",
+            "config": {"workload": "Larsen-Kolda KRPDrawSamples N=3 I=2^23 R=32 J=50000 j=N+1 (all modes)",
+                       "N": N, "I": I, "R": R, "J": J, "j": j},
+            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK, "traffic": None,
+                         "kernel": "update_sampler(k): k_gram_mfma + k_pinv_partials + k_lev_rows + tile scan",
+                         "kernel_ms": upd_ms, "algorithmic_bytes_per_update": ubytes},
+            "draw_ms": draw_ms, "update_ms_per_mode": upd_ms, "construction_s": t_build,
+            "weights_finite": bool(np.all(np.isfinite(gpu_w))),
+            "samples_in_range": bool(np.all(gpu_samples < I)),
+            "cpu_baseline": cpu}
+    line.update(cmp)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,9 +540,10 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
     ap.add_argument("--c3-ranks", default="16,64,128")
     ap.add_argument("--c3-steps", type=int, default=5)
-    ap.add_argument("--workload", choices=["krp", "als", "serve", "c3"], default="krp",
+    ap.add_argument("--workload", choices=["krp", "als", "serve", "c3", "lk"], default="krp",
                     help="krp: the KRP leverage-sampling headline (default); als: sampled CP-ALS sweeps; "
-                         "serve: the draw-request stream at several arrival rates")
+                         "serve: the draw-request stream at several arrival rates; c3: FP32 vs FP64 on "
+                         "ill-conditioned factors; lk: the Larsen-Kolda sampler")
     ap.add_argument("--als-size", choices=sorted(ALS_SIZES), default="c5")
     ap.add_argument("--als-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--als-oracle", action="store_true", help="rerun the sweeps on the CPU oracle and compare fits")
@@ -446,6 +569,9 @@ def main():
         return
     if args.workload == "c3":
         c3_main(args)
+        return
+    if args.workload == "lk":
+        lk_main(args)
         return
 
     import torch  # device buffers only; imported before libftl_hip.so so both share one HIP runtime

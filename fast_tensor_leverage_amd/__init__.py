@@ -8,9 +8,9 @@ Python-visible classes.
 """
 from .als import ALS, LowRankTensor, SparseTensor  # noqa: F401
 from .als_driver import als_prod  # noqa: F401
-from .sampler import Sampler  # noqa: F401
+from .sampler import LarsenKoldaSampler, Sampler  # noqa: F401
 from .serve import Server  # noqa: F401
 from .tensors import PyLowRank, PySparseTensor  # noqa: F401
 
-__all__ = ["Sampler", "Server", "ALS", "LowRankTensor", "SparseTensor", "PyLowRank", "PySparseTensor", "als_prod"]
+__all__ = ["Sampler", "LarsenKoldaSampler", "Server", "ALS", "LowRankTensor", "SparseTensor", "PyLowRank", "PySparseTensor", "als_prod"]
 __version__ = "0.1.0"

@@ -254,13 +254,16 @@ class ALS:
         self.J = None
 
     def initialize_ds_als(self, J: int, method: str = "efficient", precision: str = "fp64", seed=None):
-        """initialize_ds_als(J, "efficient") (als.hpp:39-55)."""
-        if method != "efficient":
-            raise ValueError(f"sampler type '{method}' is not available (only 'efficient')")
+        """initialize_ds_als(J, method) (als.hpp:39-55): method "efficient" (the
+        exact KRP leverage sampler; precision applies) or "larsen_kolda"."""
+        kinds = {"efficient": _lib.FTL_SAMPLER_EFFICIENT, "larsen_kolda": _lib.FTL_SAMPLER_LARSEN_KOLDA}
+        if method not in kinds:
+            raise ValueError(f"sampler type '{method}' is not available (provided: {sorted(kinds)})")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         L = _lib.load()
-        _lib.check(L.ftl_als_initialize_ds(self._fh._h, int(J), PRECISIONS[precision]), "initialize_ds_als")
+        _lib.check(L.ftl_als_initialize_ds_sampler(self._fh._h, int(J), kinds[method], PRECISIONS[precision]),
+                   "initialize_ds_als")
         self.J = int(J)
         if seed is not None:
             _lib.check(L.ftl_als_set_seed(self._fh._h, int(seed), 0))

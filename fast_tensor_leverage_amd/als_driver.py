@@ -11,8 +11,8 @@ from .als import ALS
 def als_prod(lhs, rhs, J, method, max_iter, stop_tolerance, epoch_length=5, verbose=False,
              precision="fp64", seed=None, host_sync=True):
     """Sweeps of ``execute_ds_als_update(j, True, True)`` (method
-    "efficient") or ``execute_exact_als_update(j, True, True)`` (method
-    "exact") over the modes, an exact fit every ``epoch_length`` sweeps,
+    "efficient" or "larsen_kolda", the sampler of initialize_ds_als) or
+    ``execute_exact_als_update(j, True, True)`` (method "exact") over the modes, an exact fit every ``epoch_length`` sweeps,
     stopped when the best of the last three fits does not beat the best before
     them by ``stop_tolerance`` (als.py:59-107).  Returns the reference's record
     (iterations, fits, update_times, fit_computation_times)."""

@@ -504,6 +504,243 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
 }
 
 // ---------------------------------------------------------------------------
+// Larsen-Kolda leverage sampler (larsen_kolda_sampler.hpp:18-136): per factor
+// k the row leverage scores of U_k (pinv of U_k^T U_k with the 1e-11 cut and
+// target rank min(I_k, R), compute_pinv, common.h:271-323), their prefix sums
+// and total; a draw samples every mode k != j independently from them.
+// ---------------------------------------------------------------------------
+// The one-block pinv kernels hold A ((R+1) R doubles) in LDS, plus V and
+// the eigenvectors when R <= 64: allow up to 132 KB of dynamic LDS.
+void pinv_lds_attrs() {
+    static bool attrs_set = false;
+    if (attrs_set) return;
+    FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      129 * 128 * 8));
+    FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      129 * 128 * 8));
+    attrs_set = true;
+}
+
+template <int RMAX>
+void launch_gram_rows(const double* X, uint64_t n, uint32_t R, double* parts, unsigned nb, hipStream_t st) {
+    hipLaunchKernelGGL((k_gram_partial<256, RMAX>), dim3(nb), dim3(256), 0, st, X, (const uint32_t*)nullptr, n, R,
+                       parts);
+    FTL_LAUNCH_CHECK();
+}
+
+void gram_rows(const double* X, uint64_t n, uint32_t R, double* parts, unsigned nb, hipStream_t st) {
+    if (R <= 8) launch_gram_rows<8>(X, n, R, parts, nb, st);
+    else if (R <= 16) launch_gram_rows<16>(X, n, R, parts, nb, st);
+    else if (R <= 32) launch_gram_rows<32>(X, n, R, parts, nb, st);
+    else if (R <= 64) launch_gram_rows<64>(X, n, R, parts, nb, st);
+    else launch_gram_rows<128>(X, n, R, parts, nb, st);
+}
+
+struct LKSampler {
+    uint32_t N = 0;
+    uint64_t J = 0, R = 0;
+    uint64_t I[kMaxModes] = {};
+    double* U[kMaxModes] = {};  // device factors (owned, or the ALS handle's)
+    double *lev[kMaxModes] = {}, *cum[kMaxModes] = {}, *off[kMaxModes] = {};
+    double *total = nullptr, *tsum = nullptr, *gparts = nullptr, *gred = nullptr, *pinv = nullptr, *ppad = nullptr,
+           *work = nullptr, *lam = nullptr;
+    // segmented tile-offset scan: thread sums, segment sums, segment offsets
+    // (+ one slot for the segment scan's total)
+    double *hsum = nullptr, *segsum = nullptr, *segoff = nullptr;
+    uint64_t nsegmax = 0;
+    uint32_t* counters = nullptr;
+    uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
+    // staging of the host-pointer API
+    uint64_t* samples_dev = nullptr;
+    double *h_dev = nullptr, *w_dev = nullptr, *draws_dev = nullptr;
+    Arena mem;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    ~LKSampler() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
+    }
+
+    // U: device factors aliased (alias) or host factors copied; st: the
+    // caller's stream (borrowed) or null for one of the sampler's own
+    void init(uint32_t N_, const uint64_t* I_, uint64_t J_, uint64_t R_, double* const* U_, bool alias,
+              hipStream_t st) {
+        FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
+        FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
+        FTL_REQUIRE(J_ >= 1 && J_ < (1ull << 40), "J must be in [1, 2^40)");
+        FTL_REQUIRE(I_ && U_, "null factor list");
+        N = N_;
+        J = J_;
+        R = R_;
+        if (st) {
+            stream = st;
+        } else {
+            FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
+            own_stream = true;
+        }
+        uint64_t imax = 1;
+        for (uint32_t k = 0; k < N; k++) {
+            FTL_REQUIRE(I_[k] >= 1 && I_[k] < (1ull << 40), "factor row counts must be in [1, 2^40)");
+            FTL_REQUIRE(U_[k], "null factor matrix");
+            I[k] = I_[k];
+            imax = std::max(imax, I[k]);
+            if (alias) {
+                U[k] = U_[k];
+            } else {
+                U[k] = mem.alloc<double>(I[k] * R);
+                FTL_HIP_CHECK(hipMemcpyAsync(U[k], U_[k], I[k] * R * sizeof(double), hipMemcpyHostToDevice, stream));
+            }
+            lev[k] = mem.alloc<double>(I[k]);
+            cum[k] = mem.alloc<double>(I[k]);
+            off[k] = mem.alloc<double>((I[k] + kLevTile - 1) / kLevTile);
+        }
+        pinv_lds_attrs();
+        total = mem.alloc<double>(N);
+        const uint64_t ntmax = (imax + kLevTile - 1) / kLevTile;
+        tsum = mem.alloc<double>(ntmax);
+        nsegmax = (ntmax + kOffSeg - 1) / kOffSeg;
+        hsum = mem.alloc<double>(nsegmax * 256);
+        segsum = mem.alloc<double>(nsegmax);
+        segoff = mem.alloc<double>(nsegmax + 1);
+        gred = mem.alloc<double>(8 * (R * (R + 1) / 2));
+        ppad = mem.alloc<double>(64 * 64);
+        gparts = mem.alloc<double>((uint64_t)kGramBlocksMax * (R * (R + 1) / 2));
+        pinv = mem.alloc<double>(R * R);
+        work = mem.alloc<double>(3 * (R + 1) * R);
+        lam = mem.alloc<double>(R);
+        counters = mem.alloc<uint32_t>(kCntSlots);
+        for (uint32_t k = 0; k < N; k++) update(k);
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // update_sampler(k) (larsen_kolda_sampler.hpp:45-100): Gram, pinv
+    // (target rank min(I_k, R)), leverage scores with their tile sums, tile
+    // offsets and total, prefix sums
+    void update(uint32_t k) {
+        FTL_REQUIRE(k < N, "update_sampler: mode index out of range");
+        const uint64_t n = I[k];
+        const uint32_t r = (uint32_t)R;
+        const unsigned nb = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, (n + 4095) / 4096));
+        if (r <= 16)
+            hipLaunchKernelGGL(k_gram_mfma<1>, dim3(nb), dim3(256), 0, stream, U[k], n, r, gparts);
+        else if (r <= 32)
+            hipLaunchKernelGGL(k_gram_mfma<2>, dim3(nb), dim3(256), 0, stream, U[k], n, r, gparts);
+        else if (r <= 48)
+            hipLaunchKernelGGL(k_gram_mfma<3>, dim3(nb), dim3(256), 0, stream, U[k], n, r, gparts);
+        else if (r <= 64)
+            hipLaunchKernelGGL(k_gram_mfma<4>, dim3(nb), dim3(256), 0, stream, U[k], n, r, gparts);
+        else
+            gram_rows(U[k], n, r, gparts, nb, stream);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, stream, counters + kCntUnique,
+                           (uint32_t)std::min<uint64_t>(n, R));
+        FTL_LAUNCH_CHECK();
+        // the nb block partials in 8 ordered chunks, then the pinv
+        const uint32_t T = r * (r + 1) / 2;
+        hipLaunchKernelGGL(k_partials_to8, dim3((8 * T + 255) / 256), dim3(256), 0, stream, gparts, nb, T, gred);
+        FTL_LAUNCH_CHECK();
+        const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
+        hipLaunchKernelGGL(k_pinv_partials<1024>, dim3(1), dim3(1024), jac * sizeof(double), stream, gred, 8u,
+                           counters, r, pinv, work, lam);
+        FTL_LAUNCH_CHECK();
+        const uint64_t nt = (n + kLevTile - 1) / kLevTile;
+        FTL_REQUIRE((nt + 3) / 4 <= 0x7fffffffull, "factor too tall for one launch");
+        // tiles of 64 rows, four per block: k_lev_rows takes one per wave
+        // (grid over all tiles); the scans stride over them
+        const unsigned gl = (unsigned)((nt + 3) / 4);
+        const unsigned g = grid_for(nt, 4, 2048);
+        if (r <= 64) {
+            const uint32_t rm = r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64;
+            hipLaunchKernelGGL(k_pad_square, dim3(1), dim3(256), 0, stream, pinv, r, rm, ppad);
+            FTL_LAUNCH_CHECK();
+            if (rm == 8)
+                hipLaunchKernelGGL(k_lev_rows<8>, dim3(gl), dim3(256), 0, stream, U[k], n, r, ppad, lev[k], tsum);
+            else if (rm == 16)
+                hipLaunchKernelGGL(k_lev_rows<16>, dim3(gl), dim3(256), 0, stream, U[k], n, r, ppad, lev[k], tsum);
+            else if (rm == 32)
+                hipLaunchKernelGGL(k_lev_rows<32>, dim3(gl), dim3(256), 0, stream, U[k], n, r, ppad, lev[k], tsum);
+            else
+                hipLaunchKernelGGL(k_lev_rows<64>, dim3(gl), dim3(256), 0, stream, U[k], n, r, ppad, lev[k], tsum);
+            FTL_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL((k_row_leverage<64, 2>), dim3(grid_for(n, 4, 16384)), dim3(256), 0, stream, U[k], n,
+                               r, pinv, lev[k]);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_tile_sums, dim3(g), dim3(256), 0, stream, lev[k], n, tsum);
+            FTL_LAUNCH_CHECK();
+        }
+        const uint64_t nseg = (nt + kOffSeg - 1) / kOffSeg;
+        hipLaunchKernelGGL(k_seg_tile_sums, dim3((unsigned)nseg), dim3(256), 0, stream, tsum, nt, hsum, segsum);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, stream, segsum, nseg, segoff, segoff + nseg);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_seg_tile_offsets, dim3((unsigned)nseg), dim3(256), 0, stream, tsum, nt, hsum, segoff,
+                           off[k], total + k);
+        FTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cum_rows, dim3(g), dim3(256), 0, stream, lev[k], n, off[k], cum[k]);
+        FTL_LAUNCH_CHECK();
+    }
+
+    // KRPDrawSamples(j) (larsen_kolda_sampler.hpp:102-133): samples (N x J,
+    // rows k != j written), h (J x R) and weights (J), each optional but
+    // samples; draws: uniform of (sample i, mode k) at draws[k J + i], or null
+    // for the built-in Philox stream
+    void draw_device(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws, hipStream_t st) {
+        FTL_REQUIRE(samples != nullptr, "null samples buffer");
+        FTL_REQUIRE(N > 1 || j >= N, "no mode left to sample (N == 1 and j == 0)");
+        if (!st) st = stream;
+        LKArgs a{};
+        for (uint32_t k = 0; k < N; k++) {
+            a.cum[k] = cum[k];
+            a.off[k] = off[k];
+            a.lev[k] = lev[k];
+            a.I[k] = I[k];
+        }
+        a.total = total;
+        a.N = (int)N;
+        a.j = j < N ? (int)j : -1;
+        a.J = J;
+        a.draws = draws;
+        a.seed = seed;
+        a.call = call;
+        a.samples = samples;
+        a.w = w;
+        hipLaunchKernelGGL(k_lk_draw, dim3(grid_for(J, 256, 16384)), dim3(256), 0, st, a);
+        FTL_LAUNCH_CHECK();
+        if (h) {
+            FactorPtrs F{};
+            for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
+            hipLaunchKernelGGL(k_fill_h, dim3(grid_for(J * R, 256, 16384)), dim3(256), 0, st, samples, J,
+                               (uint32_t)R, (int)N, a.j, F, h);
+            FTL_LAUNCH_CHECK();
+        }
+        if (!draws) call++;
+    }
+
+    void draw_host(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws) {
+        FTL_REQUIRE(samples != nullptr, "null samples array");
+        if (!samples_dev) {
+            samples_dev = mem.alloc<uint64_t>(N * J);
+            h_dev = mem.alloc<double>(J * R);
+            w_dev = mem.alloc<double>(J);
+            draws_dev = mem.alloc<double>(2 * N * J);
+        }
+        if (draws)
+            FTL_HIP_CHECK(hipMemcpyAsync(draws_dev, draws, 2 * N * J * sizeof(double), hipMemcpyHostToDevice,
+                                         stream));
+        draw_device(j, samples_dev, h ? h_dev : nullptr, w ? w_dev : nullptr, draws ? draws_dev : nullptr, stream);
+        for (uint32_t k = 0; k < N; k++)
+            if (k != j)
+                FTL_HIP_CHECK(hipMemcpyAsync(samples + (uint64_t)k * J, samples_dev + (uint64_t)k * J,
+                                             J * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        if (h) FTL_HIP_CHECK(hipMemcpyAsync(h, h_dev, J * R * sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (w) FTL_HIP_CHECK(hipMemcpyAsync(w, w_dev, J * sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+};
+
+// ---------------------------------------------------------------------------
 // LowRankTensor + ALS (als.hpp:31-220, low_rank_tensor.hpp)
 // ---------------------------------------------------------------------------
 struct Als {
@@ -520,8 +757,10 @@ struct Als {
     Arena mem;
     CubTemp cub;
     Matches mt;
-    // sampled update state (initialize_ds_als)
+    // sampled update state (initialize_ds_als): the exact sampler, or the
+    // Larsen-Kolda sampler
     Sampler* sampler = nullptr;
+    std::unique_ptr<LKSampler> lk;
     uint64_t J = 0;
     KeyLayout dlayout[kMaxModes] = {};
     uint64_t* samples = nullptr;
@@ -559,15 +798,7 @@ struct Als {
         FTL_HIP_CHECK(hipGetDevice(&device));
         // blocking stream: ordered after the caller's default-stream work
         FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
-        static bool attrs_set = false;
-        if (!attrs_set) {
-            // A ((R+1) R doubles) in LDS, plus V and the eigenvectors when R <= 64
-            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<256>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
-            FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<1024>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
-            attrs_set = true;
-        }
+        pinv_lds_attrs();
         mt.init(mem);
         for (uint32_t k = 0; k < N; k++) {
             FTL_REQUIRE(I_[k] >= 1 && I_[k] < (1ull << 32), "factor row counts must be in [1, 2^32)");
@@ -593,11 +824,13 @@ struct Als {
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
-    void initialize_ds(uint64_t J_, int precision) {
+    // kind: FTL_SAMPLER_EFFICIENT (precision applies) or FTL_SAMPLER_LARSEN_KOLDA
+    void initialize_ds(uint64_t J_, int precision, int kind = 0) {
         FTL_REQUIRE(X, "the sampled update needs a sparse tensor");
         FTL_REQUIRE(N >= 2, "the sampled update needs at least two modes");
         FTL_REQUIRE(J_ >= 1 && J_ < (1ull << 31), "J must be in [1, 2^31)");
-        FTL_REQUIRE(!sampler, "initialize_ds_als was already called");
+        FTL_REQUIRE(!sampler && !lk, "initialize_ds_als was already called");
+        FTL_REQUIRE(kind == 0 || kind == 1, "unknown sampler kind");
         for (uint32_t j = 0; j < N; j++) {
             dlayout[j] = make_layout(N, j, ibits);
             if (dlayout[j].total_bits > 64)
@@ -605,7 +838,12 @@ struct Als {
                                                   " need more than 64 key bits"};
         }
         J = J_;
-        sampler = sampler_new_aliased(N, I, J, R, U, precision);
+        if (kind == 1) {
+            lk.reset(new LKSampler());
+            lk->init(N, I, J, R, U, true, stream);
+        } else {
+            sampler = sampler_new_aliased(N, I, J, R, U, precision);
+        }
         samples = mem.alloc<uint64_t>(N * J);
         FTL_HIP_CHECK(hipMemsetAsync(samples, 0, N * J * sizeof(uint64_t), stream));
         h = mem.alloc<double>(J * R);
@@ -653,13 +891,14 @@ struct Als {
 
     // execute_ds_als_update(j, renormalize, update_sampler) (als.hpp:57-220)
     void ds_update(uint32_t j, bool renormalize, bool update_sampler, const double* draws_dev) {
-        FTL_REQUIRE(sampler, "initialize_ds_als has not been called");
+        FTL_REQUIRE(sampler || lk, "initialize_ds_als has not been called");
         FTL_REQUIRE(j < N, "mode index out of range");
         have_events = timing;
         hipStream_t st = stream;
         record(0);
         // 1. draw J samples of the KRP of the modes != j
-        sampler_draw(sampler, j, samples, h, w, draws_dev, st);
+        if (lk) lk->draw_device(j, samples, h, w, draws_dev, st);
+        else sampler_draw(sampler, j, samples, h, w, draws_dev, st);
         record(1);
         // 2. sort / unique with counts, scaled rows, sampled Gram pinv
         const KeyLayout& DL = dlayout[j];
@@ -718,7 +957,10 @@ struct Als {
                 FTL_LAUNCH_CHECK();
             }
         }
-        if (update_sampler) sampler_rebuild(sampler, j, st);
+        if (update_sampler) {
+            if (lk) lk->update(j);
+            else sampler_rebuild(sampler, j, st);
+        }
         record(4);
     }
 
@@ -1064,10 +1306,25 @@ int ftl_als_initialize_ds(ftl_als* a, uint64_t J, int precision) {
     });
 }
 
+int ftl_als_initialize_ds_sampler(ftl_als* a, uint64_t J, int sampler_type, int precision) {
+    return guarded([&] {
+        FTL_REQUIRE(a, "null handle");
+        FTL_REQUIRE(precision == FTL_FP64 || precision == FTL_FP32, "precision must be FTL_FP64 or FTL_FP32");
+        FTL_REQUIRE(sampler_type == FTL_SAMPLER_EFFICIENT || sampler_type == FTL_SAMPLER_LARSEN_KOLDA,
+                    "sampler_type must be FTL_SAMPLER_EFFICIENT or FTL_SAMPLER_LARSEN_KOLDA");
+        a->impl.initialize_ds(J, precision, sampler_type == FTL_SAMPLER_LARSEN_KOLDA ? 1 : 0);
+    });
+}
+
 int ftl_als_set_seed(ftl_als* a, uint64_t seed, uint64_t counter) {
     return guarded([&] {
-        FTL_REQUIRE(a && a->impl.sampler, "initialize_ds_als has not been called");
-        ftl::sampler_set_seed(a->impl.sampler, seed, counter);
+        FTL_REQUIRE(a && (a->impl.sampler || a->impl.lk), "initialize_ds_als has not been called");
+        if (a->impl.lk) {
+            a->impl.lk->seed = seed;
+            a->impl.lk->call = counter;
+        } else {
+            ftl::sampler_set_seed(a->impl.sampler, seed, counter);
+        }
     });
 }
 
@@ -1082,7 +1339,7 @@ int ftl_als_ds_update(ftl_als* a, uint32_t j, int renormalize, int update_sample
     return guarded([&] {
         FTL_REQUIRE(a, "null handle");
         ftl::Als& A = a->impl;
-        FTL_REQUIRE(A.sampler, "initialize_ds_als has not been called");
+        FTL_REQUIRE(A.sampler || A.lk, "initialize_ds_als has not been called");
         const double* d = nullptr;
         if (draws_host) {
             if (!A.draws_stage) A.draws_stage = A.mem.alloc<double>(2 * A.N * A.J);
@@ -1125,9 +1382,92 @@ int ftl_als_set_factor(ftl_als* a, uint32_t k, const double* U_host) {
 
 int ftl_als_update_sampler(ftl_als* a, uint32_t k) {
     return guarded([&] {
-        FTL_REQUIRE(a && a->impl.sampler, "initialize_ds_als has not been called");
-        ftl::sampler_rebuild(a->impl.sampler, k, a->impl.stream);
+        FTL_REQUIRE(a && (a->impl.sampler || a->impl.lk), "initialize_ds_als has not been called");
+        FTL_REQUIRE(k < a->impl.N, "mode index out of range");
+        if (a->impl.lk) a->impl.lk->update(k);
+        else ftl::sampler_rebuild(a->impl.sampler, k, a->impl.stream);
         FTL_HIP_CHECK(hipStreamSynchronize(a->impl.stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Larsen-Kolda sampler handle (Sampler(U, J, R, "larsen_kolda"))
+// ---------------------------------------------------------------------------
+struct ftl_lk_sampler {
+    ftl::LKSampler impl;
+};
+
+int ftl_lk_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
+                          ftl_lk_sampler** out) {
+    return guarded([&] {
+        FTL_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<ftl_lk_sampler> s(new ftl_lk_sampler());
+        s->impl.init(N, I, J, R, const_cast<double* const*>(U_host), false, nullptr);
+        *out = s.release();
+    });
+}
+
+int ftl_lk_sampler_destroy(ftl_lk_sampler* s) {
+    return guarded([&] { delete s; });
+}
+
+int ftl_lk_sampler_draw(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
+                        const double* draws) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null handle");
+        s->impl.draw_host(j, samples, h, weights, draws);
+    });
+}
+
+int ftl_lk_sampler_draw_device(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
+                               const double* draws, void* stream) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null handle");
+        s->impl.draw_device(j, samples, h, weights, draws, (hipStream_t)stream);
+    });
+}
+
+int ftl_lk_sampler_update(ftl_lk_sampler* s, uint32_t j, const double* Uj_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null handle");
+        ftl::LKSampler& L = s->impl;
+        FTL_REQUIRE(j < L.N, "update_sampler: mode index out of range");
+        if (Uj_host)
+            FTL_HIP_CHECK(hipMemcpyAsync(L.U[j], Uj_host, L.I[j] * L.R * sizeof(double), hipMemcpyHostToDevice,
+                                         L.stream));
+        L.update(j);
+        FTL_HIP_CHECK(hipStreamSynchronize(L.stream));
+    });
+}
+
+void* ftl_lk_sampler_stream(ftl_lk_sampler* s) { return s ? (void*)s->impl.stream : nullptr; }
+
+int ftl_lk_sampler_factor_device(ftl_lk_sampler* s, uint32_t k, double** out) {
+    return guarded([&] {
+        FTL_REQUIRE(s && out, "null argument");
+        FTL_REQUIRE(k < s->impl.N, "mode index out of range");
+        *out = s->impl.U[k];
+    });
+}
+
+int ftl_lk_sampler_set_seed(ftl_lk_sampler* s, uint64_t seed, uint64_t counter) {
+    return guarded([&] {
+        FTL_REQUIRE(s, "null handle");
+        s->impl.seed = seed;
+        s->impl.call = counter;
+    });
+}
+
+int ftl_lk_sampler_leverage(ftl_lk_sampler* s, uint32_t k, double* lev_host, double* total_host) {
+    return guarded([&] {
+        FTL_REQUIRE(s && lev_host, "null argument");
+        ftl::LKSampler& L = s->impl;
+        FTL_REQUIRE(k < L.N, "mode index out of range");
+        FTL_HIP_CHECK(hipMemcpyAsync(lev_host, L.lev[k], L.I[k] * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+        if (total_host)
+            FTL_HIP_CHECK(hipMemcpyAsync(total_host, L.total + k, sizeof(double), hipMemcpyDeviceToHost, L.stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(L.stream));
     });
 }
 

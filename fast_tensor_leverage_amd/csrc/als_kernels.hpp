@@ -302,8 +302,17 @@ __global__ void __launch_bounds__(BS) k_pinv_partials(const double* __restrict__
             a++;
         }
         const int b = a + (e - base);
+        // partials summed in block order; loads issued 16 at a time
         double s = 0.0;
-        for (uint32_t p = 0; p < nparts; p++) s += partial[(uint64_t)p * T + e];
+        uint32_t p = 0;
+        for (; p + 16 <= nparts; p += 16) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = partial[(uint64_t)(p + u) * T + e];
+#pragma unroll
+            for (int u = 0; u < 16; u++) s += v[u];
+        }
+        for (; p < nparts; p++) s += partial[(uint64_t)p * T + e];
         st.A[a * LD + b] = s;
         st.A[b * LD + a] = s;
     }
@@ -883,6 +892,414 @@ __global__ void __launch_bounds__(256) k_chain_final(const double* __restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Larsen-Kolda leverage sampler (larsen_kolda_sampler.hpp): per factor its
+// Gram U^T U (k_gram_mfma), the pinv of the Gram (k_pinv_partials), the row
+// leverage scores lev_i = u_i^T pinv u_i (compute_DAGAT, sampler.hpp:94-122;
+// k_lev_rows) with their sums over tiles of 64 rows, the prefix sums of the
+// scores (k_tile_offsets, k_cum_rows), and per sample and mode an independent
+// inverse-CDF draw with the log-space weight of larsen_kolda_sampler.hpp:105-128
+// (k_lk_draw).  Every sum has a fixed association, so the scores, prefix sums
+// and draws do not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLevTile = 64;  // rows of one leverage tile: one wave, one row per lane
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Gram of a tall factor (I x R, R <= 16 NB) on the FP64 matrix cores:
+// v_mfma_f64_16x16x4f64 takes A (16 x 4) with A[i][k] in lane 16k + i and
+// B (4 x 16) with B[k][j] in lane 16k + j, and accumulates D (16 x 16) with
+// D[4v + l/16][l%16] in item v of lane l.  With lane l holding
+// U[r0 + l/16][16a + l%16] as x_a, mfma(x_a, x_b) adds the four rows' share of
+// the Gram block (a, b); only blocks a <= b are formed.  Block b0 sums rows
+// [b0 chunk, (b0+1) chunk) (chunk a multiple of 16); wave w of it takes the
+// 4-row groups r0 = start + 4w + 16s; the four waves' blocks are added in
+// wave order.  partial[b0] holds the packed upper triangle (entry (r, c),
+// r <= c, at r R - r(r-1)/2 + c - r), the layout k_pinv_partials reduces.
+template <int NB>
+__global__ void __launch_bounds__(256) k_gram_mfma(const double* __restrict__ U, uint64_t I, uint32_t R,
+                                                   double* __restrict__ partial) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    constexpr int UNR = NB <= 2 ? 8 : 4;  // 16-row steps per loop trip (loads in flight)
+    __shared__ double red[4][4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t chunk = (((I + gridDim.x - 1) / gridDim.x) + 15) & ~15ull;
+    const uint64_t rb = (uint64_t)blockIdx.x * chunk;
+    const uint64_t re = min<uint64_t>(I, rb + chunk);
+    const int kr = lane >> 4, ci = lane & 15;
+    f64x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (uint64_t r0 = rb + 4 * w; r0 < re; r0 += 16 * UNR) {
+        double x[UNR][NB];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint64_t row = r0 + 16 * u + kr;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const uint32_t col = 16 * b + ci;
+                x[u][b] = (row < re && col < R) ? U[row * R + col] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            int t = 0;
+#pragma unroll
+            for (int a = 0; a < NB; a++)
+#pragma unroll
+                for (int b = a; b < NB; b++) {
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][a], x[u][b], acc[t], 0, 0, 0);
+                    t++;
+                }
+        }
+    }
+    double* out = partial + (uint64_t)blockIdx.x * (R * (R + 1) / 2);
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+        for (int b = a; b < NB; b++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[w][v][lane] = acc[t][v];
+            __syncthreads();
+            if (w == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double sum = ((red[0][v][lane] + red[1][v][lane]) + red[2][v][lane]) + red[3][v][lane];
+                    const uint32_t r = 16 * a + 4 * v + (lane >> 4), c = 16 * b + (lane & 15);
+                    if (r < R && c < R && r <= c) out[r * R - r * (r - 1) / 2 + c - r] = sum;
+                }
+            }
+            __syncthreads();
+            t++;
+        }
+}
+
+// P (R x R) into Pp (RM x RM), zero outside R x R.
+__global__ void k_pad_square(const double* __restrict__ P, uint32_t R, uint32_t RM, double* __restrict__ Pp) {
+    for (uint32_t e = threadIdx.x; e < RM * RM; e += blockDim.x) {
+        const uint32_t r = e / RM, c = e - r * RM;
+        Pp[e] = (r < R && c < R) ? P[r * R + c] : 0.0;
+    }
+}
+
+// Row leverage scores of an I x R factor (R <= RMAX), one row per lane: wave
+// w of block b takes tile t = 4b + w (64 rows; the grid covers all tiles, one
+// per wave, so the scalar loads of Pp stay in straight-line code instead of
+// being hoisted out of a tile loop into spilled registers), stages it through
+// LDS with coalesced loads (row stride RMAX + 2 doubles, so the row reads are
+// free of bank conflicts), and lane l forms
+// lev = sum_k u_k (P_kk u_k + 2 sum_{c>k} P_kc u_c) from its row in registers
+// and the wave-uniform entries of Pp (RMAX x RMAX, zero outside R x R: padded
+// columns contribute exact zeros).  tsum[t] = the inclusive 64-lane scan's
+// last value over tile t (rows past I count 0), the same scan k_cum_rows
+// takes, so the tile offsets and the prefix sums agree bitwise.
+template <int RMAX>
+__global__ void __launch_bounds__(256, 2) k_lev_rows(const double* __restrict__ U, uint64_t I, uint32_t R,
+                                                     const double* __restrict__ Pp, double* __restrict__ lev,
+                                                     double* __restrict__ tsum) {
+    constexpr int ST = RMAX + 2;
+    constexpr int RPI = 64 / RMAX;  // rows per staging load
+    __shared__ __attribute__((aligned(16))) double stage[4][kLevTile * ST];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double* tile = stage[w];
+    const uint64_t ntiles = (I + kLevTile - 1) / kLevTile;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + w;
+    if (t >= ntiles) return;
+    const int lr = lane / RMAX, lc = lane % RMAX;
+    const uint64_t row0 = t * kLevTile;
+    double v[RMAX];
+#pragma unroll
+    for (int q = 0; q < RMAX; q++) {
+        const uint64_t row = row0 + q * RPI + lr;
+        v[q] = (row < I && (uint32_t)lc < R) ? U[row * R + lc] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < RMAX; q++) tile[(q * RPI + lr) * ST + lc] = v[q];
+    wave_sync_lds();
+    double u[RMAX];
+#pragma unroll
+    for (int c = 0; c < RMAX; c += 2) {
+        const double2 p = *reinterpret_cast<const double2*>(tile + lane * ST + c);
+        u[c] = p.x;
+        u[c + 1] = p.y;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < RMAX; k++) {
+        double sk = 0.0;
+#pragma unroll
+        for (int c = k + 1; c < RMAX; c++) sk = __builtin_fma(Pp[k * RMAX + c], u[c], sk);
+        acc = __builtin_fma(u[k], __builtin_fma(Pp[k * RMAX + k], u[k], 2.0 * sk), acc);
+    }
+    const bool valid = row0 + lane < I;
+    if (valid) lev[row0 + lane] = acc;
+    const double sc = group_incl_scan<64>(valid ? acc : 0.0, lane);
+    if (lane == 63) tsum[t] = sc;
+}
+
+// Leverage scores for R > 64: R-lane groups, one row per group (lanes take
+// columns c = lane + q G; y_c = (u^T P)_c, P symmetric), group sum.
+template <int G, int CPL>
+__global__ void __launch_bounds__(256) k_row_leverage(const double* __restrict__ U, uint64_t I, uint32_t R,
+                                                      const double* __restrict__ P, double* __restrict__ lev) {
+    const uint32_t lane = threadIdx.x % G;
+    const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
+    for (uint64_t i = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; i < I; i += groups) {
+        const double* u = U + i * R;
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const uint32_t c = lane + q * G;
+            if (c < R) {
+                double y = 0.0;
+                for (uint32_t k = 0; k < R; k++) y = __builtin_fma(u[k], P[k * R + c], y);
+                part = __builtin_fma(u[c], y, part);
+            }
+        }
+        const double s = group_sum<G>(part);
+        if (lane == 0) lev[i] = s;
+    }
+}
+
+// tsum[t] for the scores of k_row_leverage: the 64-lane inclusive scan of
+// tile t's scores (rows past I count 0), last value.
+__global__ void __launch_bounds__(256) k_tile_sums(const double* __restrict__ lev, uint64_t I,
+                                                   double* __restrict__ tsum) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t ntiles = (I + kLevTile - 1) / kLevTile;
+    for (uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = t * kLevTile + lane;
+        const double sc = group_incl_scan<64>(i < I ? lev[i] : 0.0, lane);
+        if (lane == 63) tsum[t] = sc;
+    }
+}
+
+// Exclusive tile offsets off[t] = off[t-1] + tsum[t-1] (off[0] = 0), taken in
+// order: thread h of one 1024-thread block owns tiles [h per, (h+1) per); the
+// threads' sums are scanned in thread order by thread 0; every thread then
+// runs its tiles from its offset.  *total = the running value after the last
+// tile, which equals the last prefix sum k_cum_rows writes.
+__global__ void __launch_bounds__(1024) k_tile_offsets(const double* __restrict__ tsum, uint64_t ntiles,
+                                                       double* __restrict__ off, double* __restrict__ total) {
+    __shared__ double part[1024];
+    const uint64_t per = (ntiles + 1023) / 1024;
+    const uint64_t tb = min<uint64_t>(ntiles, threadIdx.x * per), te = min<uint64_t>(ntiles, tb + per);
+    // sums in tile order; loads issued 16 at a time
+    double s = 0.0;
+    uint64_t t0 = tb;
+    for (; t0 + 16 <= te; t0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = tsum[t0 + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++) s = s + v[u];
+    }
+    for (; t0 < te; t0++) s = s + tsum[t0];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int h = 0; h < 1024; h++) {
+            const double x = part[h];
+            part[h] = run;
+            run = run + x;
+        }
+    }
+    __syncthreads();
+    double run = part[threadIdx.x];
+    uint64_t t1 = tb;
+    for (; t1 + 16 <= te; t1 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = tsum[t1 + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            off[t1 + u] = run;
+            run = run + v[u];
+        }
+    }
+    for (; t1 < te; t1++) {
+        off[t1] = run;
+        run = run + tsum[t1];
+    }
+    if (tb < te && te == ntiles) *total = run;
+}
+
+// Tile offsets for many tiles, in segments of kOffSeg tiles (one block of
+// 256 threads each; thread h owns 16 consecutive tiles):
+//   k_seg_tile_sums   hsum[s 256 + h] = thread h's tiles summed in order,
+//                     segsum[s] = the 256 thread sums in thread order;
+//   k_tile_offsets    on segsum: segoff (exclusive, in segment order);
+//   k_seg_tile_offsets off[t] = segoff[s] + thread sums before h (in order)
+//                     + tiles before t in h's range (in order); *total = the
+//                     running value after the last tile (the last prefix sum
+//                     k_cum_rows writes).
+constexpr uint32_t kOffSeg = 4096;
+
+__global__ void __launch_bounds__(256) k_seg_tile_sums(const double* __restrict__ tsum, uint64_t ntiles,
+                                                       double* __restrict__ hsum, double* __restrict__ segsum) {
+    __shared__ double part[256];
+    const uint64_t tb = min<uint64_t>(ntiles, (uint64_t)blockIdx.x * kOffSeg + 16 * threadIdx.x);
+    const uint64_t te = min<uint64_t>(ntiles, tb + 16);
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = tb + u < te ? tsum[tb + u] : 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+        if (tb + u < te) s = s + v[u];
+    hsum[(uint64_t)blockIdx.x * 256 + threadIdx.x] = s;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int h = 0; h < 256; h++) run = run + part[h];
+        segsum[blockIdx.x] = run;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_seg_tile_offsets(const double* __restrict__ tsum, uint64_t ntiles,
+                                                          const double* __restrict__ hsum,
+                                                          const double* __restrict__ segoff, double* __restrict__ off,
+                                                          double* __restrict__ total) {
+    __shared__ double base[256];
+    const uint64_t tb = min<uint64_t>(ntiles, (uint64_t)blockIdx.x * kOffSeg + 16 * threadIdx.x);
+    const uint64_t te = min<uint64_t>(ntiles, tb + 16);
+    base[threadIdx.x] = hsum[(uint64_t)blockIdx.x * 256 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = segoff[blockIdx.x];
+        for (int h = 0; h < 256; h++) {
+            const double x = base[h];
+            base[h] = run;
+            run = run + x;
+        }
+    }
+    __syncthreads();
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = tb + u < te ? tsum[tb + u] : 0.0;
+    double run = base[threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+        if (tb + u < te) {
+            off[tb + u] = run;
+            run = run + v[u];
+        }
+    if (tb < te && te == ntiles) *total = run;
+}
+
+// Block partials of a packed Gram (nparts x T) reduced to 8 chunk sums
+// (out[h T + e] = partials [h c, (h+1) c) of entry e in order, c =
+// ceil(nparts / 8)), so the one-block pinv kernel reads 8 partials.
+__global__ void __launch_bounds__(256) k_partials_to8(const double* __restrict__ partial, uint32_t nparts, uint32_t T,
+                                                      double* __restrict__ out) {
+    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t e = idx >> 3, h = idx & 7;
+    if (e >= T) return;
+    const uint32_t c = (nparts + 7) / 8;
+    const uint32_t pb = min(nparts, h * c), pe = min(nparts, pb + c);
+    double s = 0.0;
+    uint32_t p = pb;
+    for (; p + 16 <= pe; p += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = partial[(uint64_t)(p + u) * T + e];
+#pragma unroll
+        for (int u = 0; u < 16; u++) s = s + v[u];
+    }
+    for (; p < pe; p++) s = s + partial[(uint64_t)p * T + e];
+    out[(uint64_t)h * T + e] = s;
+}
+
+// cum[i] = off[t] + the inclusive 64-lane scan of tile t's scores up to i.
+__global__ void __launch_bounds__(256) k_cum_rows(const double* __restrict__ lev, uint64_t I,
+                                                  const double* __restrict__ off, double* __restrict__ cum) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t ntiles = (I + kLevTile - 1) / kLevTile;
+    for (uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = t * kLevTile + lane;
+        const double sc = group_incl_scan<64>(i < I ? lev[i] : 0.0, lane);
+        if (i < I) cum[i] = off[t] + sc;
+    }
+}
+
+__global__ void k_set_u32(uint32_t* __restrict__ p, uint32_t v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+struct LKArgs {
+    const double* cum[kMaxModes];
+    const double* off[kMaxModes];  // tile offsets (cum at the end of the previous tile)
+    const double* lev[kMaxModes];
+    const double* total;  // N entries
+    uint64_t I[kMaxModes];
+    int N, j;
+    uint64_t J;
+    const double* draws;  // uniform of (sample i, mode k) at draws[k J + i], or null
+    uint64_t seed, call;
+    uint64_t* samples;    // N x J
+    double* w;            // J or null
+};
+
+// First row of mode k whose prefix sum exceeds x (I_k - 1 if none): the tile
+// from its offsets (off[t] equals the prefix sum at the end of tile t - 1, so
+// the first row past x lies in the tile before the first offset past x),
+// then the row within the tile's 64 prefix sums.
+__device__ inline uint64_t lk_search(const double* __restrict__ off, const double* __restrict__ cum, uint64_t I,
+                                     double x) {
+    const uint64_t nt = (I + kLevTile - 1) / kLevTile;
+    uint64_t lo = 1, hi = nt;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (off[mid] > x) hi = mid;
+        else lo = mid + 1;
+    }
+    const uint64_t t = lo - 1;
+    uint64_t a = t * kLevTile, b = min<uint64_t>(I, a + kLevTile);
+    while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (cum[mid] > x) b = mid;
+        else a = mid + 1;
+    }
+    return a < I ? a : I - 1;
+}
+
+// Sample i: per mode k != j, u in [0,1), the first row whose inclusive prefix
+// sum exceeds u * total (rows of zero leverage are never drawn), and the
+// weight exp(-log J + sum_k (log total_k - log lev_k[row])).
+__global__ void k_lk_draw(LKArgs a) {
+#pragma clang fp contract(off)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.J; i += (uint64_t)gridDim.x * blockDim.x) {
+        double logw = 0.0 - log((double)a.J);
+        for (int k = 0; k < a.N; k++) {
+            if (k == a.j) continue;
+            const double u = a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k);
+            const double tot = a.total[k];
+            const uint64_t row = lk_search(a.off[k], a.cum[k], a.I[k], u * tot);
+            a.samples[(uint64_t)k * a.J + i] = row;
+            logw = logw + (log(tot) - log(a.lev[k][row]));
+        }
+        if (a.w) a.w[i] = exp(logw);
+    }
+}
+
+// h = product over modes k != j of the sampled rows (fill_h_by_samples,
+// sampler.hpp:75-92)
+__global__ void k_fill_h(const uint64_t* __restrict__ samples, uint64_t J, uint32_t R, int N, int j, FactorPtrs F,
+                         double* __restrict__ h) {
+#pragma clang fp contract(off)
+    const uint64_t n = J * R;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / R, c = e - i * R;
+        double v = 1.0;
+        for (int k = 0; k < N; k++)
+            if (k != j) v = v * F.U[k][samples[(uint64_t)k * J + i] * R + c];
+        h[e] = v;
+    }
 }
 
 }  // namespace ftl

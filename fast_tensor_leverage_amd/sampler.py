@@ -57,10 +57,17 @@ class Sampler:
     and leaf scan).
     """
 
+    def __new__(cls, matrices=None, J=None, R=None, type: str = "efficient", precision: str = "fp64"):
+        # Sampler(U, J, R, "larsen_kolda") builds the Larsen-Kolda sampler
+        # (als_module.cpp:45-52)
+        if cls is Sampler and type == "larsen_kolda":
+            return LarsenKoldaSampler(matrices, J, R)
+        return super().__new__(cls)
+
     def __init__(self, matrices, J: int, R: int, type: str = "efficient", precision: str = "fp64"):
         if type != "efficient":
             raise ValueError(f"Unsupported argument {type!r} passed for sampler construction "
-                             "(only 'efficient' is provided)")
+                             "(provided: 'efficient', 'larsen_kolda')")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         if not isinstance(matrices, (list, tuple)) or len(matrices) == 0:
@@ -238,3 +245,122 @@ class Sampler:
         E = np.empty((self.N, self.R, self.R))
         _lib.check(_lib.load().ftl_sampler_compute_m(self.handle, int(j), _ptr(pinv), _ptr(E)), "compute_m")
         return pinv, E
+
+
+class LarsenKoldaSampler:
+    """Larsen-Kolda leverage sampler (larsen_kolda_sampler.hpp:18-136):
+    ``Sampler(matrices, J, R, "larsen_kolda")``.  Every mode is sampled
+    independently in proportion to its factor's row leverage scores; the
+    weights are 1 / (J prod_k p_k).  ``random_draws`` uses its first N*J
+    uniforms (mode k's at [k*J, (k+1)*J)); without it the built-in Philox
+    stream advances per call (the reference draws from mt19937)."""
+
+    def __init__(self, matrices, J: int, R: int):
+        if not isinstance(matrices, (list, tuple)) or len(matrices) == 0:
+            raise TypeError("matrices must be a non-empty list of I_k x R float64 arrays")
+        J, R = int(J), int(R)
+        if J < 1 or R < 1:
+            raise ValueError("J and R must be positive")
+        for k, U in enumerate(matrices):
+            _require_array(U, np.float64, 2, f"matrices[{k}]")
+            if U.shape[1] != R:
+                raise ValueError(f"matrices[{k}] has {U.shape[1]} columns, expected R={R}")
+        L = _lib.load()
+        _lib.require_device()
+        self.N, self.J, self.R = len(matrices), J, R
+        self.dims = [int(U.shape[0]) for U in matrices]
+        self._matrices = list(matrices)
+        dims = (ctypes.c_uint64 * self.N)(*self.dims)
+        ptrs = (ctypes.c_void_p * self.N)(*[_ptr(U) for U in matrices])
+        h = ctypes.c_void_p()
+        _lib.check(L.ftl_lk_sampler_create(self.N, dims, J, R, ptrs, ctypes.byref(h)), "Sampler(larsen_kolda)")
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().ftl_lk_sampler_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("sampler is closed")
+        return self._h
+
+    def _check_samples(self, samples):
+        _require_array(samples, np.uint64, 2, "samples")
+        if samples.shape != (self.N, self.J):
+            raise ValueError(f"samples must have shape (N, J) = {(self.N, self.J)}, got {samples.shape}")
+
+    def _draws(self, random_draws):
+        if random_draws is None:
+            return None
+        _require_array(random_draws, np.float64, 1, "random_draws")
+        if random_draws.size != 2 * self.N * self.J:
+            raise ValueError("random_draws must hold 2*N*J uniforms")
+        return _ptr(random_draws)
+
+    def KRPDrawSamples(self, j: int, samples: np.ndarray, random_draws: np.ndarray = None):
+        self._check_samples(samples)
+        if int(j) < 0:
+            raise ValueError("j must be non-negative")
+        _lib.check(_lib.load().ftl_lk_sampler_draw(self.handle, int(j), _ptr(samples), None, None,
+                                                   self._draws(random_draws)), "KRPDrawSamples")
+
+    def KRPDrawSamples_materialize(self, j: int, samples: np.ndarray, h_out: np.ndarray,
+                                   weights_out: np.ndarray, random_draws: np.ndarray = None):
+        self._check_samples(samples)
+        _require_array(h_out, np.float64, 2, "h_out")
+        _require_array(weights_out, np.float64, 1, "weights_out")
+        if h_out.shape != (self.J, self.R) or weights_out.shape != (self.J,):
+            raise ValueError("h_out must be J x R and weights_out J")
+        _lib.check(_lib.load().ftl_lk_sampler_draw(self.handle, int(j), _ptr(samples), _ptr(h_out),
+                                                   _ptr(weights_out), self._draws(random_draws)),
+                   "KRPDrawSamples_materialize")
+
+    def update_sampler(self, j: int, Uj: np.ndarray = None):
+        """Recompute factor j's leverage scores (larsen_kolda_sampler.hpp:45-100)
+        from ``Uj`` (copied in) or, without it, from the factor resident on
+        the device."""
+        if not 0 <= int(j) < self.N:
+            raise ValueError("j out of range")
+        if Uj is not None:
+            _require_array(Uj, np.float64, 2, "Uj")
+            if Uj.shape != (self.dims[j], self.R):
+                raise ValueError("Uj has the wrong shape")
+            self._matrices[j] = Uj
+        _lib.check(_lib.load().ftl_lk_sampler_update(self.handle, int(j), None if Uj is None else _ptr(Uj)),
+                   "update_sampler")
+
+    def factor_device_ptr(self, k: int) -> int:
+        out = ctypes.c_void_p()
+        _lib.check(_lib.load().ftl_lk_sampler_factor_device(self.handle, int(k), ctypes.byref(out)), "factor")
+        return int(out.value)
+
+    def stream(self) -> int:
+        """hipStream_t of the handle (update_sampler and the host draws run on it)."""
+        return int(_lib.load().ftl_lk_sampler_stream(self.handle) or 0)
+
+    def draw_device(self, j: int, samples, h=None, weights=None, draws=None, stream=None):
+        st = 0 if stream is None else (stream if isinstance(stream, int) else int(stream.cuda_stream))
+        _lib.check(_lib.load().ftl_lk_sampler_draw_device(
+            self.handle, int(j), _dev_ptr(samples, "samples"), _dev_ptr(h, "h") or None,
+            _dev_ptr(weights, "weights") or None, _dev_ptr(draws, "draws") or None, st or None), "draw_device")
+
+    def set_seed(self, seed: int, counter: int = 0):
+        _lib.check(_lib.load().ftl_lk_sampler_set_seed(self.handle, int(seed) & (2**64 - 1), int(counter)),
+                   "set_seed")
+
+    def leverage(self, k: int):
+        """(leverage scores of factor k, their total)."""
+        out = np.empty(self.dims[k])
+        tot = np.zeros(1)
+        _lib.check(_lib.load().ftl_lk_sampler_leverage(self.handle, int(k), _ptr(out), _ptr(tot)), "leverage")
+        return out, float(tot[0])

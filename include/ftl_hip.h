@@ -39,6 +39,12 @@ typedef enum ftl_precision {
     FTL_FP32 = 1  /* FP64 top tree levels, FP32 bottom levels and leaf scan */
 } ftl_precision;
 
+/* Sampler kinds of ALS::initialize_ds_als (als.hpp:39-55, als_module.cpp:34-62). */
+typedef enum ftl_sampler_type {
+    FTL_SAMPLER_EFFICIENT = 0,    /* the exact KRP leverage sampler ("efficient") */
+    FTL_SAMPLER_LARSEN_KOLDA = 1  /* per-factor leverage product ("larsen_kolda") */
+} ftl_sampler_type;
+
 typedef struct ftl_sampler ftl_sampler;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -237,6 +243,12 @@ int ftl_als_destroy(ftl_als* a);
  * the handle's device factors (shared, not copied). */
 int ftl_als_initialize_ds(ftl_als* a, uint64_t J, int precision);
 
+/* initialize_ds_als(J, method) with the method as an ftl_sampler_type
+ * (als.hpp:39-55): FTL_SAMPLER_EFFICIENT is ftl_als_initialize_ds;
+ * FTL_SAMPLER_LARSEN_KOLDA builds the Larsen-Kolda sampler on the handle's
+ * factors (precision does not apply to it). */
+int ftl_als_initialize_ds_sampler(ftl_als* a, uint64_t J, int sampler_type, int precision);
+
 /* Seed and counter of the sampler's built-in uniforms. */
 int ftl_als_set_seed(ftl_als* a, uint64_t seed, uint64_t counter);
 
@@ -300,6 +312,42 @@ int ftl_als_enable_timing(ftl_als* a, int enable);
 
 /* Stream of the handle (hipStream_t). */
 void* ftl_als_stream(ftl_als* a);
+
+/* ------------------------------------------------------------------------
+ * Larsen-Kolda leverage sampler -- replaces LarsenKoldaSampler
+ * (larsen_kolda_sampler.hpp:18-136; Sampler(U, J, R, "larsen_kolda"),
+ * als_module.cpp:45-52): per factor the row leverage scores
+ * u_i^T pinv(U^T U) u_i (compute_DAGAT, sampler.hpp:94-122; pinv with the
+ * 1e-11 cut and target rank min(I_k, R)), and per draw every mode k != j
+ * sampled independently in proportion to them, with weight
+ * exp(-log J + sum_k (log total_k - log lev_k[row])).  The reference draws from
+ * std::discrete_distribution on mt19937; here a row is the first whose
+ * inclusive prefix sum exceeds u * total, u from draws[k J + i] when given
+ * (the first N J of the 2 N J uniforms) or the built-in Philox stream.
+ * ------------------------------------------------------------------------ */
+typedef struct ftl_lk_sampler ftl_lk_sampler;
+
+int ftl_lk_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
+                          ftl_lk_sampler** out);
+int ftl_lk_sampler_destroy(ftl_lk_sampler* s);
+/* KRPDrawSamples / KRPDrawSamples_materialize: samples N x J (rows k != j),
+ * h J x R and weights J optional (NULL), draws optional; returns when done. */
+int ftl_lk_sampler_draw(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
+                        const double* draws);
+/* Same on device buffers, enqueued on stream (NULL: the handle's). */
+int ftl_lk_sampler_draw_device(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
+                               const double* draws, void* stream);
+/* update_sampler(j) (larsen_kolda_sampler.hpp:45-100) from a new factor j
+ * (I_j x R, copied in), or with Uj_host NULL from the resident factor j (as
+ * written through ftl_lk_sampler_factor_device).  Returns when done. */
+int ftl_lk_sampler_update(ftl_lk_sampler* s, uint32_t j, const double* Uj_host);
+/* Device address of the sampler's factor k (I_k x R FP64). */
+int ftl_lk_sampler_factor_device(ftl_lk_sampler* s, uint32_t k, double** out);
+/* Stream of the handle (hipStream_t). */
+void* ftl_lk_sampler_stream(ftl_lk_sampler* s);
+int ftl_lk_sampler_set_seed(ftl_lk_sampler* s, uint64_t seed, uint64_t counter);
+/* Leverage scores of factor k (I_k) and their total (optional). */
+int ftl_lk_sampler_leverage(ftl_lk_sampler* s, uint32_t k, double* lev_host, double* total_host);
 
 /* ------------------------------------------------------------------------
  * Draw-request server (no counterpart in the reference, whose only surface is

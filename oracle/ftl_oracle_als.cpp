@@ -273,12 +273,31 @@ uint64_t ftlo_als_dedup(const uint64_t* samples, uint64_t N, uint64_t J, uint64_
 // arrays the oracle sampler aliases; U[j] is overwritten in place, sigma
 // receives the new column norms when renormalize is set.  diag (optional,
 // 4 entries): J', found_count, 0, 0.
+int ftlo_als_update_drawn(void* sparse, double* const* U, const uint64_t* dims, uint64_t N, uint64_t J, uint64_t R,
+                          uint64_t j, const uint64_t* samples_in, const double* h_in, const double* w_in,
+                          int renormalize, double* sigma, double* diag);
+
 int ftlo_als_update(void* sampler, void* sparse, double* const* U, const uint64_t* dims, uint64_t N, uint64_t J,
                     uint64_t R, uint64_t j, const double* draws, int renormalize, int update_sampler, double* sigma,
                     double* diag) {
     std::vector<uint64_t> samples(N * J, 0);
     std::vector<double> h(J * R), w(J);
     ftlo_sampler_draw(sampler, j, draws, samples.data(), h.data(), w.data(), nullptr, nullptr);
+    const int rc = ftlo_als_update_drawn(sparse, U, dims, N, J, R, j, samples.data(), h.data(), w.data(), renormalize,
+                                         sigma, diag);
+    if (rc == 0 && update_sampler) ftlo_sampler_update(sampler, j, U[j]);
+    return rc;
+}
+
+// The update of execute_ds_als_update after the draw (als.hpp:76-220): the
+// samples (N x J), h (J x R) and weights (J) of any sampler -- the efficient
+// sampler above, the Larsen-Kolda sampler -- go through dedup, pinv, sampled
+// MTTKRP, solve and renormalize.  The caller updates its sampler.
+int ftlo_als_update_drawn(void* sparse, double* const* U, const uint64_t* dims, uint64_t N, uint64_t J, uint64_t R,
+                          uint64_t j, const uint64_t* samples_in, const double* h_in, const double* w_in,
+                          int renormalize, double* sigma, double* diag) {
+    std::vector<uint64_t> samples(samples_in, samples_in + N * J);
+    std::vector<double> h(h_in, h_in + J * R), w(w_in, w_in + J);
     std::vector<uint64_t> first(J), count(J), tuples(J * N);
     const uint64_t Jp = ftlo_als_dedup(samples.data(), N, J, j, first.data(), count.data(), tuples.data(), 0, 0, 0);
     std::vector<double> hd(Jp * R), wd(Jp);
@@ -310,7 +329,6 @@ int ftlo_als_update(void* sampler, void* sparse, double* const* U, const uint64_
             Uj[i * R + c] = acc;
         }
     if (renormalize) ftlo_renormalize_columns(Uj, Ij, R, sigma);
-    if (update_sampler) ftlo_sampler_update(sampler, j, Uj);
     if (diag) {
         diag[0] = (double)Jp;
         diag[1] = (double)found;

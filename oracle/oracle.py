@@ -39,6 +39,16 @@ built here: it needs cblas.h/lapacke.h):
                           (3) sweeps drive the fit of a noisy tensor to ~1
   exact_mttkrp            the dense MTTKRP X_(j) (KRP of the other factors)
                           of the densified tensor (numpy einsum)
+  lk_update / leverage    leverage = squared row norms of the thin QR factor
+  (Larsen-Kolda)          (numpy.linalg.qr) for full-rank factors; the rank
+                          deficient case against the projector of the
+                          numpy pinv; total = rank (tests/test_oracle_lk.py)
+  lk_draw                 row = searchsorted(cumsum(lev), u * total,
+                          'right') in NumPy; weights = 1 / (J prod_k p_k) with
+                          p_k = lev_k[row] / total_k; h = product of the
+                          sampled rows; empirical row frequencies against
+                          lev / total (chi-square)
+  als_update_drawn        same checks as als_update, fed by either sampler
   als_exact_update        the dense least-squares solution argmin ||X_(j) -
                           U_j KRP^T|| (numpy lstsq); exact recovery of a
                           noise-free factor; the fit never decreases over a
@@ -121,6 +131,16 @@ def _als_bind(L):
     L.ftlo_sparse_exact_mttkrp.argtypes = [_vp, _u64, _vp, _u64, _u64, _vp]
     L.ftlo_als_exact_update.argtypes = [_vp, _vp, _vp, _u64, _u64, _u64, ctypes.c_int, _vp]
     L.ftlo_als_exact_update.restype = ctypes.c_int
+    L.ftlo_als_update_drawn.argtypes = [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                        _vp]
+    L.ftlo_als_update_drawn.restype = ctypes.c_int
+    L.ftlo_lk_create.argtypes = [_vp, _vp, _u64, _u64, _u64]
+    L.ftlo_lk_create.restype = _vp
+    L.ftlo_lk_destroy.argtypes = [_vp]
+    L.ftlo_lk_update.argtypes = [_vp, _u64, _vp]
+    L.ftlo_lk_leverage.argtypes = [_vp, _u64, _vp, _vp]
+    L.ftlo_lk_draw.argtypes = [_vp, _u64, _vp, _vp, _vp, _vp, _vp]
+    L.ftlo_lk_draw.restype = ctypes.c_int
 
 
 def _p(a):
@@ -367,6 +387,79 @@ def als_dedup(samples, j):
     tuples = np.empty((J, N), dtype=np.uint64)
     u = int(lib().ftlo_als_dedup(_p(samples), N, J, j, _p(first), _p(count), _p(tuples), None, None, None))
     return first[:u], count[:u], tuples[:u]
+
+
+class OracleLKSampler:
+    """LarsenKoldaSampler (larsen_kolda_sampler.hpp:18-136) with injected
+    uniforms: mode k of sample i uses draws[k*J + i].  The factors are
+    copied into ``self.U`` and aliased by the C restatement."""
+
+    def __init__(self, U_list, J: int, R: int):
+        self.U = [_c(u, np.float64).copy() for u in U_list]
+        self.N, self.J, self.R = len(self.U), int(J), int(R)
+        self._I = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
+        self._h = lib().ftlo_lk_create(_ptrs(self.U), _p(self._I), self.N, self.J, self.R)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ftlo_lk_destroy(h)
+            self._h = None
+
+    def update_sampler(self, j: int, Uj=None):
+        if Uj is not None:
+            self.U[j] = _c(Uj, np.float64).copy()
+        lib().ftlo_lk_update(self._h, j, _p(self.U[j]))
+
+    def leverage(self, k: int):
+        lev = np.empty(self.U[k].shape[0])
+        tot = np.zeros(1)
+        lib().ftlo_lk_leverage(self._h, k, _p(lev), _p(tot))
+        return lev, float(tot[0])
+
+    def draw(self, j, draws, samples=None):
+        draws = _c(draws, np.float64)
+        if draws.size < self.N * self.J:
+            raise ValueError("draws must hold at least N*J uniforms")
+        samples = np.zeros((self.N, self.J), dtype=np.uint64) if samples is None else samples
+        h = np.empty((self.J, self.R))
+        w = np.empty(self.J)
+        m = np.empty(self.J)
+        lib().ftlo_lk_draw(self._h, j, _p(draws), _p(samples), _p(h), _p(w), _p(m))
+        return {"samples": samples, "h": h, "weights": w, "margins": m}
+
+
+class OracleLKALS:
+    """ALS(LowRankTensor(R, U), SparseTensor) + initialize_ds_als(J,
+    "larsen_kolda") restated: the Larsen-Kolda draw followed by the same
+    update as the efficient sampler's (als.hpp:57-220)."""
+
+    def __init__(self, sparse: OracleSparse, U_list, J: int, sigma=None):
+        R = U_list[0].shape[1]
+        self.X = sparse
+        self.sampler = OracleLKSampler(U_list, J, R)
+        self.U = self.sampler.U
+        self.N, self.J, self.R = len(self.U), int(J), R
+        self.sigma = np.ones(R) if sigma is None else _c(sigma, np.float64).copy()
+
+    def update(self, j, draws, renormalize=True, update_sampler=True):
+        d = self.sampler.draw(j, draws)
+        dims = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
+        diag = np.zeros(4)
+        rc = lib().ftlo_als_update_drawn(self.X._h, _ptrs(self.U), _p(dims), self.N, self.J, self.R, j,
+                                         _p(d["samples"]), _p(d["h"]), _p(d["weights"]), int(renormalize),
+                                         _p(self.sigma), _p(diag))
+        if rc != 0:
+            raise RuntimeError("oracle ALS update failed")
+        if update_sampler:
+            self.sampler.update_sampler(j)
+        return {"unique_samples": int(diag[0]), "found": int(diag[1]), "margins": d["margins"]}
+
+    def residual_normsq(self) -> float:
+        return self.X.residual_normsq(self.sigma, self.U)
+
+    def fit(self) -> float:
+        return 1.0 - np.sqrt(self.residual_normsq()) / np.sqrt(self.X.normsq())
 
 
 class OracleALS:

@@ -151,3 +151,12 @@ def sampled_ls_solution(X, U, j, samples, h, w):
     mt = (cols * w[None, :]) @ h
     G = (h * w[:, None]).T @ h
     return mt @ np.linalg.pinv(G, rcond=1e-13, hermitian=True)
+
+
+def lowrank_as_sparse(U):
+    """The dense CP tensor [[U]] (X) and all its entries as a sparse tensor
+    (idx uint32 nnz x N, values)."""
+    dims = [M.shape[0] for M in U]
+    X = cp_dense(U)
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), axis=-1).reshape(-1, len(dims))
+    return X, np.ascontiguousarray(idx.astype(np.uint32)), X.reshape(-1).copy()

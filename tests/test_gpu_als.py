@@ -221,8 +221,10 @@ def test_als_errors(ftl):
     with pytest.raises(ValueError):
         als.execute_ds_als_update(0, True, True)
     with pytest.raises(ValueError):
-        als.initialize_ds_als(100, "larsen_kolda")
+        als.initialize_ds_als(100, "uniform")
     als.initialize_ds_als(100)
+    with pytest.raises(ValueError):
+        als.initialize_ds_als(100, "larsen_kolda")  # already initialised
     with pytest.raises(ValueError):
         als.execute_ds_als_update(3, True, True)
     with pytest.raises(ValueError):

@@ -5,7 +5,7 @@ noise-free low-rank tensor."""
 import numpy as np
 import pytest
 
-from tests.helpers import (brute_mttkrp, cp_dense, dense_from_sparse, random_sparse,
+from tests.helpers import (brute_mttkrp, cp_dense, dense_from_sparse, lowrank_as_sparse, random_sparse,
                            sampled_ls_solution)
 
 
@@ -104,13 +104,6 @@ def test_dedup_matches_numpy_unique(oracle, j):
     np.testing.assert_array_equal(first, ifirst)
 
 
-def _lowrank_as_sparse(U):
-    dims = [M.shape[0] for M in U]
-    X = cp_dense(U)
-    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), axis=-1).reshape(-1, len(dims))
-    return X, np.ascontiguousarray(idx.astype(np.uint32)), X.reshape(-1).copy()
-
-
 @pytest.mark.parametrize("j", [0, 1, 2])
 def test_als_update_recovers_exact_lowrank(oracle, j):
     """X = [[U_true]] exactly; with the other factors at their true values one
@@ -118,7 +111,7 @@ def test_als_update_recovers_exact_lowrank(oracle, j):
     rng = np.random.default_rng(40 + j)
     dims, R, J = (12, 10, 14), 4, 300
     Ut = [rng.standard_normal((d, R)) for d in dims]
-    X, idx, vals = _lowrank_as_sparse(Ut)
+    X, idx, vals = lowrank_as_sparse(Ut)
     S = oracle.OracleSparse(idx, vals)
     U0 = [u.copy() for u in Ut]
     U0[j] = rng.standard_normal((dims[j], R))
@@ -160,7 +153,7 @@ def test_als_sweeps_raise_fit(oracle):
     rng = np.random.default_rng(9)
     dims, R, J = (14, 13, 12), 3, 400
     Ut = [rng.standard_normal((d, R)) for d in dims]
-    X, idx, vals = _lowrank_as_sparse(Ut)
+    X, idx, vals = lowrank_as_sparse(Ut)
     vals = vals + 0.01 * np.linalg.norm(vals) / np.sqrt(vals.size) * rng.standard_normal(vals.size)
     als = oracle.OracleALS(oracle.OracleSparse(idx, vals), [rng.standard_normal((d, R)) for d in dims], J)
     f0 = als.fit()
