@@ -468,6 +468,20 @@ def lk_main(args):
     gpu_w = d_w.cpu().numpy().copy()
     log(f"LK: draw {draw_ms * 1e3:.1f} us ({J / (draw_ms * 1e-3):,.0f} samples/s); update_sampler "
         f"{upd_ms:.3f} ms per mode ({achieved / 1e9:.0f} GB/s)")
+    # the hybrid variant on the same factors (its draw synchronises: the
+    # candidate count sizes the enumeration)
+    del s
+    hs = ftl.Sampler(U, J, R, "larsen_kolda_hybrid")
+    hs.draw_device(j, d_s, d_h, d_w, d_draws, st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hs.draw_device(j, d_s, d_h, d_w, d_draws, st)
+    torch.cuda.synchronize()
+    hyb_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    hyb_info = hs.last_info()
+    del hs
+    log(f"LK hybrid: draw {hyb_ms:.3f} ms (host-clocked, synchronising); {hyb_info}")
     cpu = None
     cmp = {}
     if not args.skip_cpu:
@@ -520,6 +534,7 @@ def lk_main(args):
                          "kernel": "update_sampler(k): k_gram_mfma + k_pinv_partials + k_lev_rows + tile scan",
                          "kernel_ms": upd_ms, "algorithmic_bytes_per_update": ubytes},
             "draw_ms": draw_ms, "update_ms_per_mode": upd_ms, "construction_s": t_build,
+            "hybrid_draw_ms": hyb_ms, "hybrid_last_draw": hyb_info,
             "weights_finite": bool(np.all(np.isfinite(gpu_w))),
             "samples_in_range": bool(np.all(gpu_samples < I)),
             "cpu_baseline": cpu}

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libftl_hip.so")
 
 FTL_OK, FTL_EINVAL, FTL_EHIP, FTL_ENOMEM, FTL_EUNSUPPORTED = 0, 1, 2, 3, 4
 FTL_FP64, FTL_FP32 = 0, 1
-FTL_SAMPLER_EFFICIENT, FTL_SAMPLER_LARSEN_KOLDA = 0, 1
+FTL_SAMPLER_EFFICIENT, FTL_SAMPLER_LARSEN_KOLDA, FTL_SAMPLER_LARSEN_KOLDA_HYBRID = 0, 1, 2
 
 _u32, _u64, _i32, _vp, _dp = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
 
@@ -59,6 +59,8 @@ SIGNATURES = {
     "ftl_als_initialize_ds_sampler": ([_vp, _u64, _i32, _i32], _i32),
     "ftl_lk_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _vp], _i32),
     "ftl_lk_sampler_destroy": ([_vp], _i32),
+    "ftl_lkh_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _vp], _i32),
+    "ftl_lk_sampler_info": ([_vp, _vp], _i32),
     "ftl_lk_sampler_draw": ([_vp, _u32, _vp, _vp, _vp, _vp], _i32),
     "ftl_lk_sampler_draw_device": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp], _i32),
     "ftl_lk_sampler_update": ([_vp, _u32, _vp], _i32),

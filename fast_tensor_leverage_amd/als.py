@@ -255,8 +255,10 @@ class ALS:
 
     def initialize_ds_als(self, J: int, method: str = "efficient", precision: str = "fp64", seed=None):
         """initialize_ds_als(J, method) (als.hpp:39-55): method "efficient" (the
-        exact KRP leverage sampler; precision applies) or "larsen_kolda"."""
-        kinds = {"efficient": _lib.FTL_SAMPLER_EFFICIENT, "larsen_kolda": _lib.FTL_SAMPLER_LARSEN_KOLDA}
+        exact KRP leverage sampler; precision applies), "larsen_kolda" or
+        "larsen_kolda_hybrid"."""
+        kinds = {"efficient": _lib.FTL_SAMPLER_EFFICIENT, "larsen_kolda": _lib.FTL_SAMPLER_LARSEN_KOLDA,
+                 "larsen_kolda_hybrid": _lib.FTL_SAMPLER_LARSEN_KOLDA_HYBRID}
         if method not in kinds:
             raise ValueError(f"sampler type '{method}' is not available (provided: {sorted(kinds)})")
         if precision not in PRECISIONS:

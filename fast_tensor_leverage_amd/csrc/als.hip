@@ -536,6 +536,13 @@ void gram_rows(const double* X, uint64_t n, uint32_t R, double* parts, unsigned 
     else launch_gram_rows<128>(X, n, R, parts, nb, st);
 }
 
+// rows of the hybrid sampler's candidate lists: lev[i] >= *cut
+struct LevAtLeast {
+    const double* lev;
+    const double* cut;
+    __device__ bool operator()(const uint64_t& i) const { return lev[i] >= *cut; }
+};
+
 struct LKSampler {
     uint32_t N = 0;
     uint64_t J = 0, R = 0;
@@ -548,6 +555,16 @@ struct LKSampler {
     // (+ one slot for the segment scan's total)
     double *hsum = nullptr, *segsum = nullptr, *segoff = nullptr;
     uint64_t nsegmax = 0;
+    // hybrid sampler (larsen_kolda_hybrid.hpp): alpha_k, per-draw cuts,
+    // candidate lists, enumeration block counts / offsets / p sums, the
+    // deterministic count and mass, the rejection failure count
+    bool hybrid = false;
+    static constexpr uint64_t kHybBlocksMax = 1ull << 20;  // candidates <= 2^32
+    double *alpha = nullptr, *cuts = nullptr, *bmax = nullptr, *hpp = nullptr, *hpoff = nullptr, *pdet = nullptr;
+    uint64_t *cand[kMaxModes] = {}, *ncand = nullptr, *hoff = nullptr, *ndet = nullptr;
+    uint32_t *hcnt = nullptr, *hfail = nullptr;
+    CubTemp sel;
+    double info[4] = {};  // last draw: deterministic samples, their mass, candidates, 1 if enumeration skipped
     uint32_t* counters = nullptr;
     uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
     // staging of the host-pointer API
@@ -565,8 +582,9 @@ struct LKSampler {
     // U: device factors aliased (alias) or host factors copied; st: the
     // caller's stream (borrowed) or null for one of the sampler's own
     void init(uint32_t N_, const uint64_t* I_, uint64_t J_, uint64_t R_, double* const* U_, bool alias,
-              hipStream_t st) {
+              hipStream_t st, bool hybrid_ = false) {
         FTL_REQUIRE(N_ >= 1 && N_ <= (uint32_t)kMaxModes, "N must be in [1, 16]");
+        hybrid = hybrid_;
         FTL_REQUIRE(R_ >= 1 && R_ <= 128, "R must be in [1, 128]");
         FTL_REQUIRE(J_ >= 1 && J_ < (1ull << 40), "J must be in [1, 2^40)");
         FTL_REQUIRE(I_ && U_, "null factor list");
@@ -605,6 +623,25 @@ struct LKSampler {
         segoff = mem.alloc<double>(nsegmax + 1);
         gred = mem.alloc<double>(8 * (R * (R + 1) / 2));
         ppad = mem.alloc<double>(64 * 64);
+        if (hybrid) {
+            FTL_REQUIRE(imax < (1ull << 31), "the hybrid sampler needs factor row counts below 2^31");
+            alpha = mem.alloc<double>(N);
+            cuts = mem.alloc<double>(N);
+            bmax = mem.alloc<double>(kGramBlocksMax);
+            ncand = mem.alloc<uint64_t>(N);
+            for (uint32_t k = 0; k < N; k++) cand[k] = mem.alloc<uint64_t>(I[k]);
+            hcnt = mem.alloc<uint32_t>(kHybBlocksMax);
+            hoff = mem.alloc<uint64_t>(kHybBlocksMax);
+            hpp = mem.alloc<double>(kHybBlocksMax);
+            hpoff = mem.alloc<double>(kHybBlocksMax);
+            pdet = mem.alloc<double>(1);
+            ndet = mem.alloc<uint64_t>(1);
+            hfail = mem.alloc<uint32_t>(1);
+            size_t bytes = 0;
+            FTL_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, hipcub::CountingInputIterator<uint64_t>(0),
+                                                   cand[0], ncand, (int)imax, LevAtLeast{lev[0], cuts}, stream));
+            sel.ensure(bytes);
+        }
         gparts = mem.alloc<double>((uint64_t)kGramBlocksMax * (R * (R + 1) / 2));
         pinv = mem.alloc<double>(R * R);
         work = mem.alloc<double>(3 * (R + 1) * R);
@@ -680,6 +717,14 @@ struct LKSampler {
         FTL_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_cum_rows, dim3(g), dim3(256), 0, stream, lev[k], n, off[k], cum[k]);
         FTL_LAUNCH_CHECK();
+        if (hybrid) {
+            // alpha_k = max lev / total (larsen_kolda_hybrid.hpp:101-116)
+            const unsigned nbm = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, n / 16384));
+            hipLaunchKernelGGL(k_block_max, dim3(nbm), dim3(256), 0, stream, lev[k], n, bmax);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(64), 0, stream, bmax, nbm, total, k, alpha);
+            FTL_LAUNCH_CHECK();
+        }
     }
 
     // KRPDrawSamples(j) (larsen_kolda_sampler.hpp:102-133): samples (N x J,
@@ -706,7 +751,10 @@ struct LKSampler {
         a.call = call;
         a.samples = samples;
         a.w = w;
-        hipLaunchKernelGGL(k_lk_draw, dim3(grid_for(J, 256, 16384)), dim3(256), 0, st, a);
+        if (hybrid)
+            draw_hybrid(a, st);
+        else
+            hipLaunchKernelGGL(k_lk_draw, dim3(grid_for(J, 256, 16384)), dim3(256), 0, st, a);
         FTL_LAUNCH_CHECK();
         if (h) {
             FactorPtrs F{};
@@ -716,6 +764,91 @@ struct LKSampler {
             FTL_LAUNCH_CHECK();
         }
         if (!draws) call++;
+        if (hybrid) {
+            // the deterministic count, its mass and the rejection failures
+            // are checked once the draw is done
+            uint64_t nd = 0;
+            double pd = 0.0;
+            uint32_t nf = 0;
+            FTL_HIP_CHECK(hipMemcpyAsync(&nd, ndet, sizeof(nd), hipMemcpyDeviceToHost, st));
+            FTL_HIP_CHECK(hipMemcpyAsync(&pd, pdet, sizeof(pd), hipMemcpyDeviceToHost, st));
+            FTL_HIP_CHECK(hipMemcpyAsync(&nf, hfail, sizeof(nf), hipMemcpyDeviceToHost, st));
+            FTL_HIP_CHECK(hipStreamSynchronize(st));
+            info[0] = (double)nd;
+            info[1] = pd;
+            FTL_REQUIRE(nd <= J, "hybrid sampler: more than J tuples reach probability 1/J");
+            if (nf)
+                throw Error{FTL_EINVAL, "hybrid sampler: " + std::to_string(nf) + " samples still rejected after " +
+                                            std::to_string(kHybMaxAttempts) +
+                                            " attempts (the deterministic tuples hold almost all the mass)"};
+        }
+    }
+
+    // KRPDrawSamples of the hybrid sampler (larsen_kolda_hybrid.hpp:139-246):
+    // cuts, candidate lists, the candidate count on the host (it sizes the
+    // enumeration), the kept tuples in candidate order, then rejection
+    // sampling of the rest
+    void draw_hybrid(const LKArgs& a, hipStream_t st) {
+        const int j = a.j;
+        hipLaunchKernelGGL(k_hyb_cuts, dim3(1), dim3(64), 0, st, alpha, total, (int)N, j, J, cuts);
+        FTL_LAUNCH_CHECK();
+        for (uint32_t k = 0; k < N; k++) {
+            if ((int)k == j) continue;
+            size_t bytes = sel.bytes;
+            FTL_HIP_CHECK(hipcub::DeviceSelect::If(sel.p, bytes, hipcub::CountingInputIterator<uint64_t>(0), cand[k],
+                                                   ncand + k, (int)I[k], LevAtLeast{lev[k], cuts + k}, st));
+        }
+        uint64_t nc[kMaxModes] = {};
+        FTL_HIP_CHECK(hipMemcpyAsync(nc, ncand, N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        FTL_HIP_CHECK(hipStreamSynchronize(st));
+        double logc = 0.0;
+        uint64_t C = 1;
+        bool zero = false;
+        for (uint32_t k = 0; k < N; k++) {
+            if ((int)k == j) continue;
+            if (nc[k] == 0) zero = true;
+            logc += std::log((double)nc[k]);
+        }
+        // the reference skips the deterministic part (with a warning) when
+        // the candidate count reaches 2^62 (larsen_kolda_hybrid.hpp:171-173)
+        const bool skip = !zero && logc >= 62.0 * std::log(2.0);
+        if (zero) C = 0;
+        if (!zero && !skip) {
+            FTL_REQUIRE(logc < 32.0 * std::log(2.0) + 1e-9, "hybrid sampler: more than 2^32 candidate tuples");
+            for (uint32_t k = 0; k < N; k++)
+                if ((int)k != j) C *= nc[k];
+            FTL_REQUIRE(C <= (1ull << 32), "hybrid sampler: more than 2^32 candidate tuples");
+        }
+        info[2] = zero ? 0.0 : std::exp(logc);
+        info[3] = skip ? 1.0 : 0.0;
+        HybArgs ha{};
+        for (uint32_t k = 0; k < N; k++) {
+            ha.cand[k] = cand[k];
+            ha.lev[k] = lev[k];
+        }
+        ha.ncand = ncand;
+        ha.total = total;
+        ha.N = (int)N;
+        ha.j = j;
+        ha.J = J;
+        ha.C = skip ? 0 : C;
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, hfail, 0u);
+        FTL_LAUNCH_CHECK();
+        if (ha.C > 0) {
+            const unsigned nb = (unsigned)((ha.C + kHybChunk - 1) / kHybChunk);
+            hipLaunchKernelGGL(k_hyb_count, dim3(nb), dim3(256), 0, st, ha, hcnt, hpp);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_hyb_offsets, dim3(1), dim3(64), 0, st, hcnt, nb, hoff, ndet);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, st, hpp, (uint64_t)nb, hpoff, pdet);
+            FTL_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_hyb_write, dim3(nb), dim3(256), 0, st, ha, hoff, a.samples, a.w);
+            FTL_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_hyb_zero, dim3(1), dim3(64), 0, st, ndet, pdet);
+            FTL_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_hyb_reject, dim3(grid_for(J, 256, 16384)), dim3(256), 0, st, a, ndet, pdet, hfail);
     }
 
     void draw_host(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws) {
@@ -830,7 +963,7 @@ struct Als {
         FTL_REQUIRE(N >= 2, "the sampled update needs at least two modes");
         FTL_REQUIRE(J_ >= 1 && J_ < (1ull << 31), "J must be in [1, 2^31)");
         FTL_REQUIRE(!sampler && !lk, "initialize_ds_als was already called");
-        FTL_REQUIRE(kind == 0 || kind == 1, "unknown sampler kind");
+        FTL_REQUIRE(kind == 0 || kind == 1 || kind == 2, "unknown sampler kind");
         for (uint32_t j = 0; j < N; j++) {
             dlayout[j] = make_layout(N, j, ibits);
             if (dlayout[j].total_bits > 64)
@@ -838,9 +971,9 @@ struct Als {
                                                   " need more than 64 key bits"};
         }
         J = J_;
-        if (kind == 1) {
+        if (kind == 1 || kind == 2) {
             lk.reset(new LKSampler());
-            lk->init(N, I, J, R, U, true, stream);
+            lk->init(N, I, J, R, U, true, stream, kind == 2);
         } else {
             sampler = sampler_new_aliased(N, I, J, R, U, precision);
         }
@@ -1310,9 +1443,11 @@ int ftl_als_initialize_ds_sampler(ftl_als* a, uint64_t J, int sampler_type, int 
     return guarded([&] {
         FTL_REQUIRE(a, "null handle");
         FTL_REQUIRE(precision == FTL_FP64 || precision == FTL_FP32, "precision must be FTL_FP64 or FTL_FP32");
-        FTL_REQUIRE(sampler_type == FTL_SAMPLER_EFFICIENT || sampler_type == FTL_SAMPLER_LARSEN_KOLDA,
-                    "sampler_type must be FTL_SAMPLER_EFFICIENT or FTL_SAMPLER_LARSEN_KOLDA");
-        a->impl.initialize_ds(J, precision, sampler_type == FTL_SAMPLER_LARSEN_KOLDA ? 1 : 0);
+        FTL_REQUIRE(sampler_type == FTL_SAMPLER_EFFICIENT || sampler_type == FTL_SAMPLER_LARSEN_KOLDA ||
+                        sampler_type == FTL_SAMPLER_LARSEN_KOLDA_HYBRID,
+                    "sampler_type must be FTL_SAMPLER_EFFICIENT, FTL_SAMPLER_LARSEN_KOLDA or "
+                    "FTL_SAMPLER_LARSEN_KOLDA_HYBRID");
+        a->impl.initialize_ds(J, precision, sampler_type);
     });
 }
 
@@ -1405,6 +1540,24 @@ int ftl_lk_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R,
         std::unique_ptr<ftl_lk_sampler> s(new ftl_lk_sampler());
         s->impl.init(N, I, J, R, const_cast<double* const*>(U_host), false, nullptr);
         *out = s.release();
+    });
+}
+
+int ftl_lkh_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
+                           ftl_lk_sampler** out) {
+    return guarded([&] {
+        FTL_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<ftl_lk_sampler> s(new ftl_lk_sampler());
+        s->impl.init(N, I, J, R, const_cast<double* const*>(U_host), false, nullptr, true);
+        *out = s.release();
+    });
+}
+
+int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out4) {
+    return guarded([&] {
+        FTL_REQUIRE(s && out4, "null argument");
+        for (int i = 0; i < 4; i++) out4[i] = s->impl.info[i];
     });
 }
 

@@ -1287,6 +1287,227 @@ __global__ void k_lk_draw(LKArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Larsen-Kolda hybrid sampler (larsen_kolda_hybrid.hpp): tau = 1/J; alpha_k =
+// max_i lev_k[i] / total_k; the tuples whose product probability
+// p = prod_{k != j} lev_k / total_k reaches tau are taken deterministically
+// (weight 1), the other samples by rejection from the product distribution
+// (weight (1 - p_det) / (J p), p_det the mass of the deterministic tuples).
+// Candidates of mode k are the rows with lev_k >= (tau alpha_k / alpha*) total_k
+// (alpha* = prod_{k != j} alpha_k), listed in ascending row order; tuple c
+// of the candidate product has mixed-radix digits over k != j ascending (the
+// lowest mode varies fastest).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kHybChunk = 4096;        // candidates per enumeration block (16 rounds of 256)
+constexpr uint32_t kHybMaxAttempts = 65536; // rejection attempts per sample
+
+struct HybArgs {
+    const uint64_t* cand[kMaxModes];
+    const uint64_t* ncand;  // N
+    const double* lev[kMaxModes];
+    const double* total;    // N
+    int N, j;
+    uint64_t J;
+    uint64_t C;             // candidates (product of ncand over k != j)
+};
+
+// per block: maxima of lev over a contiguous chunk
+__global__ void __launch_bounds__(256) k_block_max(const double* __restrict__ v, uint64_t n,
+                                                   double* __restrict__ part) {
+    __shared__ double red[256];
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = (uint64_t)blockIdx.x * chunk, e = min<uint64_t>(n, b + chunk);
+    double m = -INFINITY;
+    for (uint64_t i = b + threadIdx.x; i < e; i += 256) m = fmax(m, v[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// alpha[k] = max over the block maxima / total[k]
+__global__ void k_alpha(const double* __restrict__ part, uint32_t nb, const double* __restrict__ total, uint32_t k,
+                        double* __restrict__ alpha) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double m = -INFINITY;
+    for (uint32_t b = 0; b < nb; b++) m = fmax(m, part[b]);
+    alpha[k] = m / total[k];
+}
+
+// cut[k] = (tau alpha_k / alpha*) total_k for k != j (larsen_kolda_hybrid.hpp:150-165)
+__global__ void k_hyb_cuts(const double* __restrict__ alpha, const double* __restrict__ total, int N, int j,
+                           uint64_t J, double* __restrict__ cut) {
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double tau = 1.0 / (double)J;
+    double astar = 1.0;
+    for (int k = 0; k < N; k++)
+        if (k != j) astar = astar * alpha[k];
+    for (int k = 0; k < N; k++) cut[k] = k == j ? 0.0 : tau * alpha[k] / astar * total[k];
+}
+
+// rows of tuple c; log p of the tuple (sum over k != j of log lev - log total)
+__device__ inline double hyb_tuple(const HybArgs& a, uint64_t c, uint64_t* rows) {
+#pragma clang fp contract(off)
+    double wlog = 0.0;
+    for (int k = 0; k < a.N; k++) {
+        if (k == a.j) continue;
+        const uint64_t nk = a.ncand[k];
+        const uint64_t d = c % nk;
+        c /= nk;
+        const uint64_t row = a.cand[k][d];
+        rows[k] = row;
+        wlog = wlog + (log(a.lev[k][row]) - log(a.total[k]));
+    }
+    return wlog;
+}
+
+// Fixed-shape exclusive scan of 256 thread values in LDS (Hillis-Steele on
+// a double buffer); returns the exclusive prefix, *tot the block total.
+__device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* buf, uint32_t* tot) {
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t x = t >= o ? buf[t - o] : 0u;
+        __syncthreads();
+        buf[t] += x;
+        __syncthreads();
+    }
+    const uint32_t incl = buf[t];
+    *tot = buf[255];
+    __syncthreads();
+    return incl - v;
+}
+
+// Pass 1 over the candidates: block b takes [b kHybChunk, (b+1) kHybChunk)
+// in 16 rounds of 256; cnt[b] = tuples kept (p >= tau); pp[b] = their p
+// summed per round by a fixed tree, rounds in order.
+__global__ void __launch_bounds__(256) k_hyb_count(HybArgs a, uint32_t* __restrict__ cnt, double* __restrict__ pp) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    const double tau = 1.0 / (double)a.J;
+    uint32_t kept = 0;
+    double psum = 0.0;
+    for (int r = 0; r < (int)(kHybChunk / 256); r++) {
+        const uint64_t c = (uint64_t)blockIdx.x * kHybChunk + (uint64_t)r * 256 + threadIdx.x;
+        double p = 0.0;
+        if (c < a.C) {
+            uint64_t rows[kMaxModes];
+            const double e = exp(hyb_tuple(a, c, rows));
+            if (e >= tau) {
+                p = e;
+                kept++;
+            }
+        }
+        red[threadIdx.x] = p;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) psum = psum + red[0];
+        __syncthreads();
+    }
+    __shared__ uint32_t kc[256];
+    kc[threadIdx.x] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int t = 0; t < 256; t++) s += kc[t];
+        cnt[blockIdx.x] = s;
+        pp[blockIdx.x] = psum;
+    }
+}
+
+// off[b] = exclusive sum of cnt (in block order); *ndet = the total
+__global__ void k_hyb_offsets(const uint32_t* __restrict__ cnt, uint32_t nb, uint64_t* __restrict__ off,
+                              uint64_t* __restrict__ ndet) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t run = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+        off[b] = run;
+        run += cnt[b];
+    }
+    *ndet = run;
+}
+
+// Pass 2: the kept tuples in candidate order at positions off[b] + ...;
+// rows k != j written (positions < J only), weight 1.
+__global__ void __launch_bounds__(256) k_hyb_write(HybArgs a, const uint64_t* __restrict__ off,
+                                                   uint64_t* __restrict__ samples, double* __restrict__ w) {
+    __shared__ uint32_t buf[256];
+    const double tau = 1.0 / (double)a.J;
+    uint64_t base = off[blockIdx.x];
+    for (int r = 0; r < (int)(kHybChunk / 256); r++) {
+        const uint64_t c = (uint64_t)blockIdx.x * kHybChunk + (uint64_t)r * 256 + threadIdx.x;
+        uint64_t rows[kMaxModes];
+        bool keep = false;
+        if (c < a.C) keep = exp(hyb_tuple(a, c, rows)) >= tau;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_u32(keep ? 1u : 0u, buf, &tot);
+        if (keep) {
+            const uint64_t pos = base + ex;
+            if (pos < a.J) {
+                for (int k = 0; k < a.N; k++)
+                    if (k != a.j) samples[(uint64_t)k * a.J + pos] = rows[k];
+                if (w) w[pos] = 1.0;
+            }
+        }
+        base += tot;
+    }
+}
+
+__global__ void k_hyb_zero(uint64_t* __restrict__ ndet, double* __restrict__ pdet) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *ndet = 0;
+        *pdet = 0.0;
+    }
+}
+
+// Samples i >= ndet: tuples from the product distribution until p < tau
+// (larsen_kolda_hybrid.hpp:213-241); weight (1 - p_det) exp(-log J - log p).
+// Attempt 0 takes the uniform of (i, k) from draws (when given) or the
+// built-in stream at stage 2k; attempt a >= 1 the built-in stream at stage
+// 2k of call (call XOR a 2^40).  A sample still rejected after
+// kHybMaxAttempts attempts keeps its last tuple and counts in *fail.
+__global__ void k_hyb_reject(LKArgs a, const uint64_t* __restrict__ ndet, const double* __restrict__ pdet,
+                             uint32_t* __restrict__ fail) {
+#pragma clang fp contract(off)
+    const double tau = 1.0 / (double)a.J;
+    const uint64_t nd = *ndet;
+    const double scale = 1.0 - *pdet;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.J; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < nd) continue;
+        double wlog = 0.0;
+        uint32_t att = 0;
+        for (;; att++) {
+            wlog = 0.0;
+            for (int k = 0; k < a.N; k++) {
+                if (k == a.j) continue;
+                double u;
+                if (att == 0)
+                    u = a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k);
+                else
+                    u = philox_uniform(a.seed, a.call ^ ((uint64_t)att << 40), i, 2 * k);
+                const double tot = a.total[k];
+                const uint64_t row = lk_search(a.off[k], a.cum[k], a.I[k], u * tot);
+                a.samples[(uint64_t)k * a.J + i] = row;
+                wlog = wlog + (log(a.lev[k][row]) - log(tot));
+            }
+            if (exp(wlog) < tau) break;
+            if (att + 1 >= kHybMaxAttempts) {
+                atomicAdd(fail, 1u);
+                break;
+            }
+        }
+        if (a.w) a.w[i] = scale * exp((0.0 - log((double)a.J)) - wlog);
+    }
+}
+
 // h = product over modes k != j of the sampled rows (fill_h_by_samples,
 // sampler.hpp:75-92)
 __global__ void k_fill_h(const uint64_t* __restrict__ samples, uint64_t J, uint32_t R, int N, int j, FactorPtrs F,

@@ -62,12 +62,14 @@ class Sampler:
         # (als_module.cpp:45-52)
         if cls is Sampler and type == "larsen_kolda":
             return LarsenKoldaSampler(matrices, J, R)
+        if cls is Sampler and type == "larsen_kolda_hybrid":
+            return LarsenKoldaSampler(matrices, J, R, hybrid=True)
         return super().__new__(cls)
 
     def __init__(self, matrices, J: int, R: int, type: str = "efficient", precision: str = "fp64"):
         if type != "efficient":
             raise ValueError(f"Unsupported argument {type!r} passed for sampler construction "
-                             "(provided: 'efficient', 'larsen_kolda')")
+                             "(provided: 'efficient', 'larsen_kolda', 'larsen_kolda_hybrid')")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         if not isinstance(matrices, (list, tuple)) or len(matrices) == 0:
@@ -253,9 +255,15 @@ class LarsenKoldaSampler:
     independently in proportion to its factor's row leverage scores; the
     weights are 1 / (J prod_k p_k).  ``random_draws`` uses its first N*J
     uniforms (mode k's at [k*J, (k+1)*J)); without it the built-in Philox
-    stream advances per call (the reference draws from mt19937)."""
+    stream advances per call (the reference draws from mt19937).
 
-    def __init__(self, matrices, J: int, R: int):
+    ``hybrid=True`` (``Sampler(matrices, J, R, "larsen_kolda_hybrid")``,
+    larsen_kolda_hybrid.hpp:20-248): the tuples of probability >= 1/J come
+    first with weight 1, the rest by rejection from the product distribution
+    with weight (1 - p_det) / (J p); ``last_info()`` reports the
+    deterministic count and mass of the last draw."""
+
+    def __init__(self, matrices, J: int, R: int, hybrid: bool = False):
         if not isinstance(matrices, (list, tuple)) or len(matrices) == 0:
             raise TypeError("matrices must be a non-empty list of I_k x R float64 arrays")
         J, R = int(J), int(R)
@@ -273,7 +281,9 @@ class LarsenKoldaSampler:
         dims = (ctypes.c_uint64 * self.N)(*self.dims)
         ptrs = (ctypes.c_void_p * self.N)(*[_ptr(U) for U in matrices])
         h = ctypes.c_void_p()
-        _lib.check(L.ftl_lk_sampler_create(self.N, dims, J, R, ptrs, ctypes.byref(h)), "Sampler(larsen_kolda)")
+        self.hybrid = bool(hybrid)
+        create = L.ftl_lkh_sampler_create if self.hybrid else L.ftl_lk_sampler_create
+        _lib.check(create(self.N, dims, J, R, ptrs, ctypes.byref(h)), "Sampler(larsen_kolda)")
         self._h = h
 
     def close(self):
@@ -357,6 +367,14 @@ class LarsenKoldaSampler:
     def set_seed(self, seed: int, counter: int = 0):
         _lib.check(_lib.load().ftl_lk_sampler_set_seed(self.handle, int(seed) & (2**64 - 1), int(counter)),
                    "set_seed")
+
+    def last_info(self) -> dict:
+        """Hybrid sampler, last draw: deterministic samples, their probability
+        mass, candidate tuples, whether the enumeration was skipped."""
+        out = np.zeros(4)
+        _lib.check(_lib.load().ftl_lk_sampler_info(self.handle, _ptr(out)), "last_info")
+        return {"deterministic": int(out[0]), "p_det": float(out[1]), "candidates": float(out[2]),
+                "skipped": bool(out[3])}
 
     def leverage(self, k: int):
         """(leverage scores of factor k, their total)."""

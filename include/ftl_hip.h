@@ -42,7 +42,8 @@ typedef enum ftl_precision {
 /* Sampler kinds of ALS::initialize_ds_als (als.hpp:39-55, als_module.cpp:34-62). */
 typedef enum ftl_sampler_type {
     FTL_SAMPLER_EFFICIENT = 0,    /* the exact KRP leverage sampler ("efficient") */
-    FTL_SAMPLER_LARSEN_KOLDA = 1  /* per-factor leverage product ("larsen_kolda") */
+    FTL_SAMPLER_LARSEN_KOLDA = 1,  /* per-factor leverage product ("larsen_kolda") */
+    FTL_SAMPLER_LARSEN_KOLDA_HYBRID = 2  /* deterministic + rejection ("larsen_kolda_hybrid") */
 } ftl_sampler_type;
 
 typedef struct ftl_sampler ftl_sampler;
@@ -330,6 +331,26 @@ typedef struct ftl_lk_sampler ftl_lk_sampler;
 int ftl_lk_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
                           ftl_lk_sampler** out);
 int ftl_lk_sampler_destroy(ftl_lk_sampler* s);
+/* Hybrid variant -- replaces LarsenKoldaHybrid (larsen_kolda_hybrid.hpp:20-248;
+ * Sampler(U, J, R, "larsen_kolda_hybrid"), als_module.cpp:45-50): tau = 1/J,
+ * alpha_k = max lev_k / total_k; per draw the rows of mode k with
+ * lev_k >= (tau alpha_k / prod_{k' != j} alpha_k') total_k are candidates
+ * (ascending row order); the tuples of the candidate product (lowest mode's
+ * digit fastest) whose probability p = prod lev_k / total_k reaches tau come
+ * first, in that order, with weight 1; the other samples are drawn from the
+ * product distribution until p < tau, weight (1 - p_det) / (J p), p_det the
+ * kept tuples' mass.  Attempt 0 of sample i takes draws[k J + i] (or the
+ * built-in stream at stage 2k), attempt a >= 1 the built-in stream at stage
+ * 2k of call (call XOR a 2^40).  Limits (the reference has none): at most
+ * 2^32 candidate tuples (the reference skips the deterministic part only from
+ * 2^62), 65536 attempts per sample, factor row counts below 2^31; exceeding
+ * them fails the draw.  Draws with this handle synchronise its stream. */
+int ftl_lkh_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
+                           ftl_lk_sampler** out);
+/* Last draw of a hybrid handle: out[0] deterministic samples, out[1] their
+ * probability mass, out[2] candidate tuples, out[3] 1 if the deterministic
+ * part was skipped (2^62 candidates or more). */
+int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out4);
 /* KRPDrawSamples / KRPDrawSamples_materialize: samples N x J (rows k != j),
  * h J x R and weights J optional (NULL), draws optional; returns when done. */
 int ftl_lk_sampler_draw(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,

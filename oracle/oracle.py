@@ -136,6 +136,11 @@ def _als_bind(L):
     L.ftlo_als_update_drawn.restype = ctypes.c_int
     L.ftlo_lk_create.argtypes = [_vp, _vp, _u64, _u64, _u64]
     L.ftlo_lk_create.restype = _vp
+    L.ftlo_lk_create2.argtypes = [_vp, _vp, _u64, _u64, _u64, ctypes.c_int]
+    L.ftlo_lk_create2.restype = _vp
+    L.ftlo_lkh_draw.argtypes = [_vp, _u64, _vp, _u64, _vp, _vp, _vp, _vp]
+    L.ftlo_lkh_draw.restype = ctypes.c_int
+    L.ftlo_lk_alpha.argtypes = [_vp, _vp]
     L.ftlo_lk_destroy.argtypes = [_vp]
     L.ftlo_lk_update.argtypes = [_vp, _u64, _vp]
     L.ftlo_lk_leverage.argtypes = [_vp, _u64, _vp, _vp]
@@ -394,11 +399,12 @@ class OracleLKSampler:
     uniforms: mode k of sample i uses draws[k*J + i].  The factors are
     copied into ``self.U`` and aliased by the C restatement."""
 
-    def __init__(self, U_list, J: int, R: int):
+    def __init__(self, U_list, J: int, R: int, hybrid: bool = False):
         self.U = [_c(u, np.float64).copy() for u in U_list]
         self.N, self.J, self.R = len(self.U), int(J), int(R)
+        self.hybrid = bool(hybrid)
         self._I = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
-        self._h = lib().ftlo_lk_create(_ptrs(self.U), _p(self._I), self.N, self.J, self.R)
+        self._h = lib().ftlo_lk_create2(_ptrs(self.U), _p(self._I), self.N, self.J, self.R, int(self.hybrid))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -417,6 +423,30 @@ class OracleLKSampler:
         lib().ftlo_lk_leverage(self._h, k, _p(lev), _p(tot))
         return lev, float(tot[0])
 
+    def alpha(self):
+        out = np.empty(self.N)
+        lib().ftlo_lk_alpha(self._h, _p(out))
+        return out
+
+    def draw_hybrid(self, j, utab):
+        """LarsenKoldaHybrid::KRPDrawSamples (larsen_kolda_hybrid.hpp:139-246):
+        utab (A x N x J) holds the uniform of attempt a, mode k, sample i."""
+        if not self.hybrid:
+            raise ValueError("not a hybrid sampler")
+        utab = _c(utab, np.float64)
+        A = utab.shape[0]
+        assert utab.shape == (A, self.N, self.J)
+        samples = np.zeros((self.N, self.J), dtype=np.uint64)
+        h = np.empty((self.J, self.R))
+        w = np.empty(self.J)
+        info = np.zeros(8)
+        rc = lib().ftlo_lkh_draw(self._h, j, _p(utab), A, _p(samples), _p(h), _p(w), _p(info))
+        if rc != 0:
+            raise RuntimeError(f"oracle hybrid draw failed ({rc})")
+        return {"samples": samples, "h": h, "weights": w, "deterministic": int(info[0]), "p_det": info[1],
+                "candidates": info[2], "skipped": bool(info[3]), "cut_margin": info[4], "keep_margin": info[5],
+                "draw_margin": info[6], "max_attempts": int(info[7])}
+
     def draw(self, j, draws, samples=None):
         draws = _c(draws, np.float64)
         if draws.size < self.N * self.J:
@@ -434,16 +464,18 @@ class OracleLKALS:
     "larsen_kolda") restated: the Larsen-Kolda draw followed by the same
     update as the efficient sampler's (als.hpp:57-220)."""
 
-    def __init__(self, sparse: OracleSparse, U_list, J: int, sigma=None):
+    def __init__(self, sparse: OracleSparse, U_list, J: int, sigma=None, hybrid: bool = False):
         R = U_list[0].shape[1]
         self.X = sparse
-        self.sampler = OracleLKSampler(U_list, J, R)
+        self.sampler = OracleLKSampler(U_list, J, R, hybrid=hybrid)
         self.U = self.sampler.U
         self.N, self.J, self.R = len(self.U), int(J), R
         self.sigma = np.ones(R) if sigma is None else _c(sigma, np.float64).copy()
 
-    def update(self, j, draws, renormalize=True, update_sampler=True):
-        d = self.sampler.draw(j, draws)
+    def update(self, j, draws, renormalize=True, update_sampler=True, utab=None):
+        """draws: the Larsen-Kolda uniforms; for the hybrid sampler utab
+        (A x N x J) gives the uniforms of every attempt instead."""
+        d = self.sampler.draw_hybrid(j, utab) if self.sampler.hybrid else self.sampler.draw(j, draws)
         dims = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
         diag = np.zeros(4)
         rc = lib().ftlo_als_update_drawn(self.X._h, _ptrs(self.U), _p(dims), self.N, self.J, self.R, j,
@@ -453,7 +485,7 @@ class OracleLKALS:
             raise RuntimeError("oracle ALS update failed")
         if update_sampler:
             self.sampler.update_sampler(j)
-        return {"unique_samples": int(diag[0]), "found": int(diag[1]), "margins": d["margins"]}
+        return {"unique_samples": int(diag[0]), "found": int(diag[1]), "margins": d.get("margins"), "draw": d}
 
     def residual_normsq(self) -> float:
         return self.X.residual_normsq(self.sigma, self.U)

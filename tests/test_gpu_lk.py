@@ -228,3 +228,113 @@ def test_lk_errors(ftl):
     one = ftl.Sampler([U[0]], 100, 4, "larsen_kolda")
     with pytest.raises(ValueError):
         one.KRPDrawSamples(0, np.zeros((1, 100), dtype=np.uint64))  # no mode left to sample
+
+
+# ---- hybrid sampler (larsen_kolda_hybrid.hpp) ----
+
+def _utab(draws, seed, call, N, J, A):
+    """Uniforms of every attempt as the hybrid draw takes them: attempt 0 from
+    draws[k J + i], attempt a >= 1 from the built-in stream at stage 2k of
+    call (call XOR a 2^40)."""
+    from tests.helpers import philox_uniforms
+    ut = np.empty((A, N, J))
+    ut[0] = draws[:N * J].reshape(N, J)
+    for a in range(1, A):
+        for k in range(N):
+            ut[a, k] = philox_uniforms(seed, call ^ (a << 40), J, 2 * k)
+    return ut
+
+
+HYB_CASES = [
+    ("hyb_small", (20, 15, 30), 3, 300, 2, 1),
+    ("hyb3", (200, 150, 300), 6, 3000, 2, 1),
+    ("hyb4", (40, 30, 20, 25), 4, 2000, 1, 2),
+    ("hyb_all", (60, 50, 40), 5, 4000, 4, 2),
+    ("hyb_flat", (500, 400, 300), 8, 2000, 0, 0),  # no tuple reaches 1/J: rejection only
+]
+
+
+@pytest.mark.parametrize("cid,dims,R,J,j,skew", HYB_CASES, ids=[c[0] for c in HYB_CASES])
+def test_hybrid_sampler_matches_oracle(ftl, oracle, cid, dims, R, J, j, skew):
+    print(f"[gpu] larsen-kolda hybrid case={cid}", flush=True)
+    rng = np.random.default_rng(_seed(cid))
+    N = len(dims)
+    U = [rng.standard_normal((d, R)) * rng.random((d, 1)) ** skew for d in dims]
+    s = ftl.Sampler(U, J, R, "larsen_kolda_hybrid")
+    seed, call = 1234 + len(cid), 7
+    s.set_seed(seed, call)
+    ref = oracle.OracleLKSampler(U, J, R, hybrid=True)
+    draws = rng.random(2 * N * J)
+    out = ref.draw_hybrid(j, _utab(draws, seed, call, N, J, 128))
+    if min(out["cut_margin"], out["keep_margin"], out["draw_margin"]) <= 1e-9:
+        pytest.skip("a leverage score, tuple probability or uniform within 1e-9 of a boundary (seed choice)")
+    samples = np.zeros((N, J), dtype=np.uint64)
+    h = np.zeros((J, R))
+    w = np.zeros(J)
+    s.KRPDrawSamples_materialize(j, samples, h, w, draws)
+    info = s.last_info()
+    assert info["deterministic"] == out["deterministic"]
+    if skew == 0 and cid == "hyb_flat":
+        assert info["deterministic"] == 0
+    else:
+        assert info["deterministic"] > 0
+    np.testing.assert_allclose(info["p_det"], out["p_det"], rtol=1e-12)
+    np.testing.assert_allclose(info["candidates"], out["candidates"], rtol=1e-12)
+    np.testing.assert_array_equal(samples, out["samples"])
+    np.testing.assert_allclose(w, out["weights"], rtol=1e-9)
+    np.testing.assert_array_equal(h, out["h"])
+
+
+def test_hybrid_builtin_uniforms_and_als(ftl, oracle):
+    """Built-in uniforms reproducible after set_seed; one ALS update with the
+    hybrid sampler equals the oracle's on the same uniforms."""
+    rng = np.random.default_rng(44)
+    dims, R, J = (40, 35, 30), 4, 600
+    U = [rng.standard_normal((d, R)) * rng.random((d, 1)) for d in dims]
+    s = ftl.Sampler(U, J, R, "larsen_kolda_hybrid")
+    s.set_seed(5, 0)
+    a = np.zeros((3, J), dtype=np.uint64)
+    s.KRPDrawSamples(1, a)
+    b = np.zeros((3, J), dtype=np.uint64)
+    s.KRPDrawSamples(1, b)
+    s.set_seed(5, 0)
+    c = np.zeros((3, J), dtype=np.uint64)
+    s.KRPDrawSamples(1, c)
+    np.testing.assert_array_equal(a, c)
+    assert not np.array_equal(a[:, s.last_info()["deterministic"]:], b[:, s.last_info()["deterministic"]:])
+
+    idx, vals = random_sparse(dims, 4000, rng, unique=False)
+    ref = oracle.OracleLKALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U], J, hybrid=True)
+    lr = ftl.LowRankTensor(R, [u.copy() for u in U])
+    als = ftl.ALS(lr, ftl.SparseTensor(idx, vals))
+    als.initialize_ds_als(J, "larsen_kolda_hybrid")
+    als.set_seed(9, 3)
+    for step, j in enumerate([0, 2]):
+        draws = rng.random(2 * 3 * J)
+        ut = _utab(draws, 9, 3, 3, J, 128)
+        probe = ref.sampler.draw_hybrid(j, ut)
+        if min(probe["cut_margin"], probe["keep_margin"], probe["draw_margin"]) <= 1e-9:
+            pytest.skip("boundary within 1e-9 (seed choice)")
+        d = ref.update(j, None, renormalize=True, update_sampler=True, utab=ut)
+        als.execute_ds_als_update(j, True, True, random_draws=draws)
+        st = als.last_stats()
+        assert st["unique_samples"] == d["unique_samples"] and st["found"] == d["found"], step
+        np.testing.assert_allclose(lr.U[j], ref.U[j], rtol=1e-9, atol=1e-11 * np.abs(ref.U[j]).max())
+        ref.U[j][:] = lr.U[j]
+        ref.sampler.update_sampler(j)
+
+
+def test_hybrid_als_prod_raises_fit(ftl):
+    from tests.helpers import lowrank_as_sparse
+    rng = np.random.default_rng(16)
+    dims, R = (30, 25, 20), 4
+    Ut = [rng.standard_normal((d, R)) for d in dims]
+    _, idx, vals = lowrank_as_sparse(Ut)
+    vals = vals + 0.01 * np.linalg.norm(vals) / np.sqrt(vals.size) * rng.standard_normal(vals.size)
+    rhs = ftl.PySparseTensor.from_arrays(idx, vals)
+    lhs = ftl.PyLowRank(dims, R, seed=11)
+    lhs.U = [u + 0.5 * rng.standard_normal(u.shape) for u in Ut]
+    lhs.ten = ftl.LowRankTensor(R, lhs.U)
+    f0 = lhs.compute_exact_fit(rhs)
+    rec = ftl.als_prod(lhs, rhs, 2000, "larsen_kolda_hybrid", 8, 1e-12, epoch_length=4, seed=3)
+    assert rec["fits"][-1] > 0.98 > f0, rec["fits"]

@@ -120,3 +120,93 @@ def test_lk_als_recovers_lowrank_and_raises_fit(oracle):
         for j in range(3):
             als.update(j, rng.random(2 * 3 * J))
     assert als.fit() > 0.98 > f0
+
+
+# ---- hybrid sampler (larsen_kolda_hybrid.hpp) ----
+
+def _tuple_probs(s, dims, j):
+    """p of every tuple over modes != j (lowest mode fastest) by brute force."""
+    modes = [k for k in range(len(dims)) if k != j]
+    ps = [s.leverage(k)[0] / s.leverage(k)[1] for k in modes]
+    grids = np.meshgrid(*[np.arange(dims[k]) for k in modes], indexing="ij")
+    P = np.ones(grids[0].shape)
+    for g, p in zip(grids, ps):
+        P = P * p[g]
+    return modes, grids, P
+
+
+@pytest.mark.parametrize("dims,R,J,j,skew", [((20, 15, 30), 3, 300, 2, 1), ((40, 50, 10), 4, 1000, 2, 0),
+                                             ((12, 9, 7, 6), 3, 400, 1, 2), ((30, 60), 3, 30, 0, 1)])
+def test_hybrid_deterministic_set_is_every_tuple_above_tau(oracle, dims, R, J, j, skew):
+    """The kept tuples are exactly those with p >= 1/J (brute force over all
+    tuples), each once, weight 1; p_det is their mass; the rejected samples
+    all have p < 1/J and weight (1 - p_det) / (J p)."""
+    rng = np.random.default_rng(sum(dims) + J + skew)
+    U = [rng.standard_normal((d, R)) * rng.random((d, 1)) ** skew for d in dims]
+    s = oracle.OracleLKSampler(U, J, R, hybrid=True)
+    out = s.draw_hybrid(j, rng.random((256, len(dims), J)))
+    modes, grids, P = _tuple_probs(s, dims, j)
+    tau = 1.0 / J
+    above = P >= tau
+    nd = out["deterministic"]
+    assert nd == int(above.sum()) and nd > 0
+    np.testing.assert_allclose(out["p_det"], P[above].sum(), rtol=1e-12)
+    det = {tuple(int(out["samples"][k][i]) for k in modes) for i in range(nd)}
+    ref = {tuple(int(g[idx]) for g in grids) for idx in zip(*np.nonzero(above))}
+    assert det == ref and len(det) == nd
+    assert np.all(out["weights"][:nd] == 1.0)
+    rest = np.ones(J - nd)
+    for k in modes:
+        lev, tot = s.leverage(k)
+        rest = rest * (lev[out["samples"][k][nd:].astype(np.int64)] / tot)
+    assert np.all(rest < tau)
+    np.testing.assert_allclose(out["weights"][nd:], (1 - out["p_det"]) / (J * rest), rtol=1e-11)
+    # listed in mixed-radix order of the row-sorted candidate lists, lowest
+    # mode fastest: lexicographic in (row of the highest mode, ..., lowest)
+    keys = [tuple(int(out["samples"][k][i]) for k in modes[::-1]) for i in range(nd)]
+    assert keys == sorted(keys)
+    # alpha = max leverage / total
+    for k in modes:
+        lev, tot = s.leverage(k)
+        np.testing.assert_allclose(s.alpha()[k], lev.max() / tot, rtol=1e-15)
+
+
+def test_hybrid_rejected_samples_follow_the_conditional_distribution(oracle):
+    """Over 60 draws, the rejection samples' tuple frequencies match
+    p / (1 - p_det) on the tuples below tau (chi-square)."""
+    rng = np.random.default_rng(31)
+    dims, R, J, j = (30, 40, 5), 3, 1000, 2
+    U = [rng.standard_normal((d, R)) * rng.random((d, 1)) for d in dims]
+    s = oracle.OracleLKSampler(U, J, R, hybrid=True)
+    modes, grids, P = _tuple_probs(s, dims, j)
+    below = P < 1.0 / J
+    counts = np.zeros(P.shape)
+    total = 0
+    for _ in range(60):
+        out = s.draw_hybrid(j, rng.random((256, 3, J)))
+        nd = out["deterministic"]
+        a = out["samples"][modes[0]][nd:].astype(np.int64)
+        b = out["samples"][modes[1]][nd:].astype(np.int64)
+        np.add.at(counts, (a, b), 1)
+        total += J - nd
+    assert not np.any(counts[~below])
+    q = P[below] / P[below].sum()
+    exp = total * q
+    ok = exp > 5
+    chi2 = np.sum((counts[below][ok] - exp[ok]) ** 2 / exp[ok])
+    dof = int(ok.sum()) - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+
+
+def test_hybrid_als_step_is_sampled_least_squares(oracle):
+    rng = np.random.default_rng(5)
+    dims, R, J, j = (15, 12, 10), 4, 300, 1
+    idx, vals = random_sparse(dims, int(np.prod(dims)) // 3, rng, unique=False)
+    X = dense_from_sparse(idx, vals, dims)
+    U = [rng.standard_normal((d, R)) for d in dims]
+    ut = rng.random((256, 3, J))
+    out = oracle.OracleLKSampler([u.copy() for u in U], J, R, hybrid=True).draw_hybrid(j, ut)
+    expect = sampled_ls_solution(X, U, j, out["samples"], out["h"], out["weights"])
+    als = oracle.OracleLKALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U], J, hybrid=True)
+    als.update(j, None, renormalize=False, update_sampler=False, utab=ut)
+    np.testing.assert_allclose(als.U[j], expect, rtol=1e-8, atol=1e-10 * np.abs(expect).max())
