@@ -475,14 +475,30 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
                 my_u = (uint32_t)(pay >> 32);
                 my_v = valj[pay & 0xffffffffull];
             }
-            for (int t = 0; t < cnt; t++) {
-                const uint64_t u = (uint32_t)__shfl(my_u, t, G);
-                const double v = __shfl(my_v, t, G);
+            // the lhs rows of 8 matches are fetched before they are added (in
+            // match order, so the sum is the same)
+            for (int t0 = 0; t0 < cnt; t0 += 8) {
+                double x[8][CPL], vv[8];
 #pragma unroll
-                for (int q = 0; q < CPL; q++) {
-                    const uint32_t c = lane + q * G;
-                    if (c < R) acc[q] = acc[q] + lhs[u * R + c] * v;
+                for (int s = 0; s < 8; s++) {
+                    const int t = t0 + s < cnt ? t0 + s : cnt - 1;
+                    const uint64_t u = (uint32_t)__shfl(my_u, t, G);
+                    vv[s] = __shfl(my_v, t, G);
+#pragma unroll
+                    for (int q = 0; q < CPL; q++) {
+                        const uint32_t c = lane + q * G;
+                        x[s][q] = c < R ? lhs[u * R + c] : 0.0;
+                    }
                 }
+#pragma unroll
+                for (int s = 0; s < 8; s++)
+                    if (t0 + s < cnt) {
+#pragma unroll
+                        for (int q = 0; q < CPL; q++) {
+                            const uint32_t c = lane + q * G;
+                            if (c < R) acc[q] = acc[q] + x[s][q] * vv[s];
+                        }
+                    }
             }
         }
 #pragma unroll
@@ -655,6 +671,14 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
 #pragma unroll
     for (int q = 0; q < CPL; q++) sq[q] = 0.0;
     __syncthreads();
+    // one column per lane: the lane keeps its column of pinv in registers
+    // (zero past R), so a row's product reads only the row sums from LDS
+    constexpr bool kRegP = SOLVE && CPL == 1;
+    double pc[kRegP ? G : 1];
+    if constexpr (kRegP) {
+#pragma unroll
+        for (int k = 0; k < G; k++) pc[k] = ((uint32_t)k < R && lane < R) ? P[k * R + lane] : 0.0;
+    }
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
     // groups run independently: a group stages its row in its own LDS slot,
     // and a group is at most one wave, whose LDS accesses are ordered
@@ -684,14 +708,26 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
             for (int q = 0; q < CPL; q++) mt_s[gi][lane + q * G] = mt[q];
             wave_sync_lds();
             const uint64_t row = seg_row[d];
+            if constexpr (kRegP) {
+                // the same sequential sum over k < R as below
+                double s = 0.0;
 #pragma unroll
-            for (int q = 0; q < CPL; q++) {
-                const uint32_t c = lane + q * G;
-                if (c < R) {
-                    double s = 0.0;
-                    for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * P[k * R + c];
-                    out[row * R + c] = s;
-                    sq[q] = sq[q] + s * s;
+                for (int k = 0; k < G; k++)
+                    if ((uint32_t)k < R) s = s + mt_s[gi][k] * pc[k];
+                if (lane < R) {
+                    out[row * R + lane] = s;
+                    sq[0] = sq[0] + s * s;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint32_t c = lane + q * G;
+                    if (c < R) {
+                        double s = 0.0;
+                        for (uint32_t k = 0; k < R; k++) s = s + mt_s[gi][k] * P[k * R + c];
+                        out[row * R + c] = s;
+                        sq[q] = sq[q] + s * s;
+                    }
                 }
             }
             wave_sync_lds();  // this row's reads of mt_s before the next row's writes
