@@ -282,6 +282,90 @@ __global__ void __launch_bounds__(BS) k_gram_partial(const double* __restrict__ 
 // v >= R - target and lambda_v > 1e-11 with target = min(J', R), then
 // pinv = M' M'^T with M'[:, v] = V[:, v] / sqrt(lambda_v) (common.h:228-323).
 // One block; pinv written full symmetric (R x R).
+// pinv of a symmetric R x R matrix A (LDS, leading dimension R + 1) as its
+// inverse by Cholesky, when that equals the eigendecomposition-based pinv
+// with the 1e-11 cut at full target rank (compute_pinv_square,
+// common.h:228-269): lambda_min(A) >= 1 / ||A^-1||_F > 2e-11 certifies that
+// no eigenvalue would be cut.  Also required: ||A||_F ||A^-1||_F < 1e6, so
+// the result agrees with the eigendecomposition route to rounding.  C (R
+// (R+1) doubles) and Li (R^2) are scratch.  Right-looking Cholesky (a column
+// per step), the inverse of the factor a column per thread (forward
+// substitution), pinv = Li^T Li; every sum in a fixed order.  Returns false
+// (pinv untouched) when a pivot is not positive or a test fails; the caller
+// then runs the eigensolver.
+constexpr int kCholMaxR = 64;
+
+template <int BS>
+__device__ bool pinv_cholesky(const double* A, double* C, double* Li, int R, double* __restrict__ pinv) {
+#pragma clang fp contract(off)
+    __shared__ double red[BS];
+    __shared__ int fail;
+    const int tid = threadIdx.x, LD = R + 1;
+    for (int e = tid; e < R * R; e += BS) {
+        const int i = e / R, j = e - i * R;
+        C[i * LD + j] = A[i * LD + j];
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    for (int k = 0; k < R; k++) {
+        const double piv = C[k * LD + k];
+        __syncthreads();
+        if (!(piv > 0.0)) return false;  // uniform: every thread read the same pivot
+        const double lkk = sqrt(piv);
+        for (int i = k + 1 + tid; i < R; i += BS) C[i * LD + k] = C[i * LD + k] / lkk;
+        if (tid == 0) C[k * LD + k] = lkk;
+        __syncthreads();
+        const int m = R - k - 1;  // trailing block, lower triangle
+        for (int e = tid; e < m * m; e += BS) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            if (j <= i) C[i * LD + j] = C[i * LD + j] - C[i * LD + k] * C[j * LD + k];
+        }
+        __syncthreads();
+    }
+    // column j of L^-1 (lower): x_j = 1 / l_jj, x_i = -(sum_{m=j}^{i-1} l_im x_m) / l_ii
+    for (int j = tid; j < R; j += BS) {
+        for (int i = 0; i < j; i++) Li[i * R + j] = 0.0;
+        Li[j * R + j] = 1.0 / C[j * LD + j];
+        for (int i = j + 1; i < R; i++) {
+            double s = 0.0;
+            for (int m = j; m < i; m++) s = s + C[i * LD + m] * Li[m * R + j];
+            Li[i * R + j] = (0.0 - s) / C[i * LD + i];
+        }
+    }
+    __syncthreads();
+    // A^-1 = Li^T Li into pinv; Frobenius norms of A^-1 and A
+    double f_inv = 0.0, f_a = 0.0;
+    for (int e = tid; e < R * R; e += BS) {
+        const int a = e / R, b = e - a * R;
+        const int m0 = a > b ? a : b;
+        double s = 0.0;
+        for (int m = m0; m < R; m++) s = s + Li[m * R + a] * Li[m * R + b];
+        pinv[e] = s;
+        f_inv = f_inv + s * s;
+        const double x = A[a * LD + b];
+        f_a = f_a + x * x;
+        if (!(s == s) || s - s != 0.0) fail = 1;  // NaN or infinity
+    }
+    red[tid] = f_inv;
+    __syncthreads();
+    for (int h = BS / 2; h > 0; h >>= 1) {
+        if (tid < h) red[tid] = red[tid] + red[tid + h];
+        __syncthreads();
+    }
+    const double fi = red[0];
+    __syncthreads();
+    red[tid] = f_a;
+    __syncthreads();
+    for (int h = BS / 2; h > 0; h >>= 1) {
+        if (tid < h) red[tid] = red[tid] + red[tid + h];
+        __syncthreads();
+    }
+    const double fa = red[0];
+    const bool ok = fail == 0 && fi > 0.0 && 1.0 / sqrt(fi) > 2e-11 && sqrt(fa) * sqrt(fi) < 1e6;
+    __syncthreads();
+    return ok;  // when false the caller overwrites pinv
+}
+
 template <int BS>
 __global__ void __launch_bounds__(BS) k_pinv_partials(const double* __restrict__ partial, uint32_t nparts,
                                                       const uint32_t* __restrict__ counters, uint32_t R,
@@ -317,9 +401,13 @@ __global__ void __launch_bounds__(BS) k_pinv_partials(const double* __restrict__
         st.A[b * LD + a] = s;
     }
     __syncthreads();
-    jacobi_block<BS>(st.A, st.V, Ri, lam, st.vecs, js, diag_tmp);
     const uint32_t Jp = counters[kCntUnique];
     const int target = (int)min<uint32_t>(Jp, R);
+    // full target rank and a well-conditioned Gram: pinv = G^-1 by Cholesky
+    if (target == Ri && Ri <= kCholMaxR && 3 * LD * Ri <= kJacobiLdsDoubles &&
+        pinv_cholesky<BS>(st.A, st.V, st.vecs, Ri, pinv))
+        return;
+    jacobi_block<BS>(st.A, st.V, Ri, lam, st.vecs, js, diag_tmp);
     // M' in place of the eigenvectors
     for (int w = tid; w < Ri * Ri; w += BS) {
         const int u = w / Ri, v = w - u * Ri;

@@ -338,3 +338,32 @@ def test_hybrid_als_prod_raises_fit(ftl):
     f0 = lhs.compute_exact_fit(rhs)
     rec = ftl.als_prod(lhs, rhs, 2000, "larsen_kolda_hybrid", 8, 1e-12, epoch_length=4, seed=3)
     assert rec["fits"][-1] > 0.98 > f0, rec["fits"]
+
+
+@pytest.mark.parametrize("kind", ["well_conditioned", "ill_conditioned", "rank_deficient"])
+def test_lk_leverage_both_pinv_routes(ftl, oracle, kind):
+    """The pinv of the factor Gram goes through the certified Cholesky route
+    (well-conditioned) or the eigensolver (condition above 1e6, or a
+    rank-deficient factor whose null direction is cut); the leverage scores
+    match the oracle's (eigensolver) either way."""
+    print(f"[gpu] larsen-kolda pinv route {kind}", flush=True)
+    rng = np.random.default_rng(len(kind))
+    I, R = 3000, 12
+    if kind == "well_conditioned":
+        A = rng.standard_normal((I, R))
+    elif kind == "ill_conditioned":
+        Q, _ = np.linalg.qr(rng.standard_normal((I, R)))
+        V, _ = np.linalg.qr(rng.standard_normal((R, R)))
+        A = (Q * np.logspace(0, -4, R)) @ V.T  # Gram condition 1e8
+    else:
+        A = rng.standard_normal((I, R - 3)) @ rng.standard_normal((R - 3, R))  # rank R - 3
+    U = [A, rng.standard_normal((500, R))]
+    s = ftl.Sampler(U, 100, R, "larsen_kolda")
+    ref = oracle.OracleLKSampler(U, 100, R)
+    lev, tot = s.leverage(0)
+    rlev, rtot = ref.leverage(0)
+    scale = rlev.max()
+    np.testing.assert_allclose(lev, rlev, rtol=1e-8, atol=1e-10 * scale)
+    np.testing.assert_allclose(tot, rtot, rtol=1e-8)
+    expect_rank = R - 3 if kind == "rank_deficient" else R
+    assert abs(tot - expect_rank) < 1e-6 * R
