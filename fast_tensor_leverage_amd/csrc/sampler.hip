@@ -42,12 +42,6 @@ struct ModeHost {
     uint64_t tree32_elems = 0;
 };
 
-__global__ void k_convert_f32(const double* in, float* out, uint64_t n) {
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n;
-         w += (uint64_t)gridDim.x * blockDim.x)
-        out[w] = (float)in[w];
-}
-
 inline unsigned grid_for(uint64_t work, unsigned bs) {
     uint64_t g = (work + bs - 1) / bs;
     if (g < 1) g = 1;
@@ -234,6 +228,13 @@ struct Sampler {
         ba.R = (uint32_t)R;
         ba.ts64 = ts64;
         ba.first_leaf = m.sh.node_count - m.sh.leaf_count;
+        // FP32 samplers: the FP32 factor copy and FP32 bottom levels are
+        // written by the same kernels (the same (float) of each FP64 value)
+        const bool f32 = precision == FTL_FP32;
+        ba.U32 = f32 ? m.U32 : nullptr;
+        ba.tree32 = f32 ? m.tree32 : nullptr;
+        ba.n64 = m.n64;
+        ba.ts32 = ts32;
         // leaves per block: their row blocks within 32 KB of LDS (a power of
         // two <= 8, at least one leaf)
         const uint64_t leaf_doubles = std::max<uint64_t>(1, m.F * R);
@@ -249,7 +250,7 @@ struct Sampler {
         const uint64_t start = m.sh.nodes_before_lfill, end = ba.first_leaf;
         if (end > start) {
             hipLaunchKernelGGL(k_level_sum, grid_nodes(end - start), dim3(256), 0, st, m.tree64, start, end - start,
-                               ts64);
+                               ts64, ba.tree32, ba.n64, ts32);
             FTL_HIP_CHECK(hipGetLastError());
         }
         // levels lfill_level - 1 .. 0 are complete: three per launch
@@ -257,26 +258,15 @@ struct Sampler {
             if (lv >= 2) {
                 const uint64_t count = 1ull << (lv - 2);
                 hipLaunchKernelGGL(k_level_sum3, grid_nodes(count), dim3(256), 0, st, m.tree64, count - 1, count,
-                                   ts64);
+                                   ts64, ba.tree32, ba.n64, ts32);
                 lv -= 3;
             } else {
                 const uint64_t count = 1ull << lv;
                 hipLaunchKernelGGL(k_level_sum, grid_nodes(count), dim3(256), 0, st, m.tree64, count - 1, count,
-                                   ts64);
+                                   ts64, ba.tree32, ba.n64, ts32);
                 lv -= 1;
             }
             FTL_HIP_CHECK(hipGetLastError());
-        }
-        if (precision == FTL_FP32) {
-            const uint64_t nel = m.I * R;
-            hipLaunchKernelGGL(k_convert_f32, dim3(grid_for(nel, 256)), dim3(256), 0, st, m.U64, m.U32, nel);
-            FTL_HIP_CHECK(hipGetLastError());
-            if (m.tree32) {
-                const uint64_t nodes = m.sh.node_count - m.n64;
-                hipLaunchKernelGGL(k_tree_to_f32, grid_nodes(nodes), dim3(256), 0, st, m.tree64, m.tree32, m.n64,
-                                   nodes, ts64, ts32);
-                FTL_HIP_CHECK(hipGetLastError());
-            }
         }
     }
 

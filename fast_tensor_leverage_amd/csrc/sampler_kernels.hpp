@@ -39,7 +39,20 @@ struct BuildArgs {
     uint32_t R;
     uint32_t ts64;
     uint64_t first_leaf;  // node_count - leaf_count
+    // FP32 samplers (else null): the factor's FP32 copy, and the FP32 copy of
+    // the nodes from n64 on (ts32 floats each, zero past ts64)
+    float* U32;
+    float* tree32;
+    uint64_t n64;
+    uint32_t ts32;
 };
+
+// FP32 copy of entry e (< ts32) of node c from its FP64 value (entries past
+// ts64 are zero), when c is in the FP32 levels
+__device__ inline void store_f32(float* tree32, uint64_t n64, uint32_t ts32, uint32_t ts64, uint64_t c, uint32_t e,
+                                 double v) {
+    if (tree32 && c >= n64) tree32[(c - n64) * ts32 + e] = e < ts64 ? (float)v : 0.0f;
+}
 
 // lpb leaves per block (BS / lpb threads each).  Dynamic LDS: the leaves'
 // row blocks (lpb x F x R doubles), then the (row, column) of every packed
@@ -68,31 +81,43 @@ __global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a, uint32_t lpb) {
         r0 = leaf_index(a.sh, node) * F;
         rows = (uint32_t)min<uint64_t>(F, a.sh.n - r0);
         const double* src = a.U + r0 * R;
-        for (uint32_t e = lt; e < rows * R; e += tpl) lu[e] = src[e];
+        for (uint32_t e = lt; e < rows * R; e += tpl) {
+            const double x = src[e];
+            lu[e] = x;
+            if (a.U32) a.U32[r0 * R + e] = (float)x;
+        }
     }
     __syncthreads();
     if (!have) return;
     double* out = a.tree + node * (uint64_t)a.ts64;
-    for (uint32_t e = lt; e < a.ts64; e += tpl) {
+    const uint32_t tsw = a.tree32 && a.ts32 > a.ts64 ? a.ts32 : a.ts64;
+    for (uint32_t e = lt; e < tsw; e += tpl) {
         double acc = 0.0;
         if (e < T) {
             const uint32_t ab = pairs[e];
             const uint32_t ra = ab & 0xffu, cb = ab >> 8;
             for (uint32_t t = 0; t < rows; t++) acc = acc + lu[t * R + ra] * lu[t * R + cb];
         }
-        out[e] = acc;
+        if (e < a.ts64) out[e] = acc;
+        store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, acc);
     }
 }
 
 // Nodes [start, start + count) of one level: parent = left + right, a block
 // per node.
-__global__ void k_level_sum(double* tree, uint64_t start, uint64_t count, uint32_t ts64) {
+__global__ void k_level_sum(double* tree, uint64_t start, uint64_t count, uint32_t ts64, float* tree32,
+                            uint64_t n64, uint32_t ts32) {
+    const uint32_t tsw = tree32 && ts32 > ts64 ? ts32 : ts64;
     for (uint64_t b = blockIdx.x; b < count; b += gridDim.x) {
         const uint64_t c = start + b;
         double* p = tree + c * ts64;
         const double* l = tree + (2 * c + 1) * ts64;
         const double* r = tree + (2 * c + 2) * ts64;
-        for (uint32_t e = threadIdx.x; e < ts64; e += blockDim.x) p[e] = l[e] + r[e];
+        for (uint32_t e = threadIdx.x; e < tsw; e += blockDim.x) {
+            const double v = e < ts64 ? l[e] + r[e] : 0.0;
+            if (e < ts64) p[e] = v;
+            store_f32(tree32, n64, ts32, ts64, c, e, v);
+        }
     }
 }
 
@@ -100,23 +125,32 @@ __global__ void k_level_sum(double* tree, uint64_t start, uint64_t count, uint32
 // c = root0 + b (b < count), reads its 8 great-grandchildren and writes the 4
 // grandchildren, 2 children and the root, each the sum of its two children
 // (the same additions as three single-level passes).
-__global__ void k_level_sum3(double* tree, uint64_t root0, uint64_t count, uint32_t ts64) {
+__global__ void k_level_sum3(double* tree, uint64_t root0, uint64_t count, uint32_t ts64, float* tree32,
+                             uint64_t n64, uint32_t ts32) {
+    const uint32_t tsw = tree32 && ts32 > ts64 ? ts32 : ts64;
     for (uint64_t b = blockIdx.x; b < count; b += gridDim.x)
-    for (uint32_t e = threadIdx.x; e < ts64; e += blockDim.x) {
+    for (uint32_t e = threadIdx.x; e < tsw; e += blockDim.x) {
         const uint64_t c = root0 + b;
+        const bool in64 = e < ts64;
         double v[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = tree[(8 * c + 7 + k) * ts64 + e];
+        for (int k = 0; k < 8; k++) v[k] = in64 ? tree[(8 * c + 7 + k) * ts64 + e] : 0.0;
         double g[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             g[k] = v[2 * k] + v[2 * k + 1];
-            tree[(4 * c + 3 + k) * ts64 + e] = g[k];
+            if (in64) tree[(4 * c + 3 + k) * ts64 + e] = g[k];
+            store_f32(tree32, n64, ts32, ts64, 4 * c + 3 + k, e, g[k]);
         }
         const double c0 = g[0] + g[1], c1 = g[2] + g[3];
-        tree[(2 * c + 1) * ts64 + e] = c0;
-        tree[(2 * c + 2) * ts64 + e] = c1;
-        tree[c * ts64 + e] = c0 + c1;
+        if (in64) {
+            tree[(2 * c + 1) * ts64 + e] = c0;
+            tree[(2 * c + 2) * ts64 + e] = c1;
+            tree[c * ts64 + e] = c0 + c1;
+        }
+        store_f32(tree32, n64, ts32, ts64, 2 * c + 1, e, c0);
+        store_f32(tree32, n64, ts32, ts64, 2 * c + 2, e, c1);
+        store_f32(tree32, n64, ts32, ts64, c, e, c0 + c1);
     }
 }
 

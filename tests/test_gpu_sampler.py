@@ -306,3 +306,35 @@ def test_overlapped_compute_m_matches_serial(ftl):
         np.testing.assert_array_equal(a, b)
     for a, b in zip(outs[1][1], outs[0][1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("R", [8, 32, 64])
+def test_fp32_copies_written_by_the_build_equal_a_separate_conversion(ftl, R):
+    """The tree build writes the FP32 factor rows and bottom levels itself;
+    rebuilding the FP32 levels afterwards by a separate conversion
+    (fp32_levels reset) gives bitwise the same draws."""
+    print(f"[gpu] fused fp32 copies R={R}", flush=True)
+    dims = [3000 + R, 2000, 900]
+    U = _factors(dims, R, 200 + R)
+    J = 4000
+    s = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    draws = np.random.default_rng(R).random(2 * 3 * J)
+    a = np.zeros((3, J), dtype=np.uint64)
+    wa = np.zeros(J)
+    ha = np.zeros((J, R))
+    s.KRPDrawSamples_materialize(4, a, ha, wa, draws)
+    s.set_option("fp32_levels", 0)  # the default again: the FP32 levels are re-converted from FP64
+    b = np.zeros((3, J), dtype=np.uint64)
+    wb = np.zeros(J)
+    hb = np.zeros((J, R))
+    s.KRPDrawSamples_materialize(4, b, hb, wb, draws)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(wa, wb)
+    np.testing.assert_array_equal(ha, hb)
+    # an updated factor goes through the same fused build
+    s.update_sampler(1, U[1] * 0.5)
+    s.KRPDrawSamples_materialize(4, a, ha, wa, draws)
+    s.set_option("fp32_levels", 0)
+    s.KRPDrawSamples_materialize(4, b, hb, wb, draws)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(wa, wb)
