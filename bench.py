@@ -573,8 +573,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the sampler runs on a single MI355X in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU: the sampler runs on a single MI355X; sharding the draws and "
+              "tree builds over several GPUs is not implemented, so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "als":
         als_main(args)
