@@ -61,6 +61,8 @@ struct Sampler {
     uint32_t fp32_levels = kDefaultFp32Levels;
     // FP32 mode: leaf scans over the FP32 factor copy (option "fp32_leaf")
     uint32_t leaf32 = 1;
+    // FP32 mode: leaf Grams on the FP64 matrix cores (option "leaf_mfma")
+    uint32_t leaf_mfma = 1;
     uint64_t tree_pad = 0;  // zeroed entries after every tree (draw kernel over-read)
     std::vector<ModeHost> modes;
     // computeM outputs and scratch.  A sampler that owns its stream keeps two
@@ -235,15 +237,29 @@ struct Sampler {
         ba.tree32 = f32 ? m.tree32 : nullptr;
         ba.n64 = m.n64;
         ba.ts32 = ts32;
-        // leaves per block: their row blocks within 32 KB of LDS (a power of
-        // two <= 8, at least one leaf)
-        const uint64_t leaf_doubles = std::max<uint64_t>(1, m.F * R);
-        uint32_t lpb = 1;
-        while (lpb < 8 && 2 * lpb * leaf_doubles <= 4096) lpb *= 2;
-        const size_t lds = (size_t)lpb * leaf_doubles * sizeof(double) + tri_size(R) * sizeof(uint16_t);
-        const uint64_t lblocks = (m.sh.leaf_count + lpb - 1) / lpb;
-        FTL_REQUIRE(lblocks <= 0x7fffffffull, "too many leaves for one launch");
-        hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)lblocks), dim3(256), lds, st, ba, lpb);
+        if (f32 && leaf_mfma && R <= 64) {
+            // FP32 samplers: leaf Grams on the matrix cores, a wave per leaf
+            const uint64_t lblocks = (m.sh.leaf_count + 3) / 4;
+            FTL_REQUIRE(lblocks <= 0x7fffffffull, "too many leaves for one launch");
+            if (R <= 16)
+                hipLaunchKernelGGL(k_leaf_gram_mfma<1>, dim3((unsigned)lblocks), dim3(256), 0, st, ba);
+            else if (R <= 32)
+                hipLaunchKernelGGL(k_leaf_gram_mfma<2>, dim3((unsigned)lblocks), dim3(256), 0, st, ba);
+            else if (R <= 48)
+                hipLaunchKernelGGL(k_leaf_gram_mfma<3>, dim3((unsigned)lblocks), dim3(256), 0, st, ba);
+            else
+                hipLaunchKernelGGL(k_leaf_gram_mfma<4>, dim3((unsigned)lblocks), dim3(256), 0, st, ba);
+        } else {
+            // leaves per block: their row blocks within 32 KB of LDS (a power
+            // of two <= 8, at least one leaf); entries bitwise the oracle's
+            const uint64_t leaf_doubles = std::max<uint64_t>(1, m.F * R);
+            uint32_t lpb = 1;
+            while (lpb < 8 && 2 * lpb * leaf_doubles <= 4096) lpb *= 2;
+            const size_t lds = (size_t)lpb * leaf_doubles * sizeof(double) + tri_size(R) * sizeof(uint16_t);
+            const uint64_t lblocks = (m.sh.leaf_count + lpb - 1) / lpb;
+            FTL_REQUIRE(lblocks <= 0x7fffffffull, "too many leaves for one launch");
+            hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)lblocks), dim3(256), lds, st, ba, lpb);
+        }
         FTL_HIP_CHECK(hipGetLastError());
         auto grid_nodes = [](uint64_t n) { return dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(n, 1), 1u << 20)); };
         // internal nodes of the lfill level (parents of a partial bottom level)
@@ -322,6 +338,9 @@ struct Sampler {
         } else if (key == "fp32_leaf") {
             FTL_REQUIRE(value == 0 || value == 1, "fp32_leaf must be 0 or 1");
             leaf32 = (uint32_t)value;
+        } else if (key == "leaf_mfma") {
+            FTL_REQUIRE(value == 0 || value == 1, "leaf_mfma must be 0 or 1");
+            leaf_mfma = (uint32_t)value;
         } else if (key == "fp32_levels") {
             FTL_REQUIRE(value >= 0 && value <= 64, "fp32_levels must be in [0, 64]");
             fp32_levels = value == 0 ? kDefaultFp32Levels : (uint32_t)value;
@@ -343,6 +362,7 @@ struct Sampler {
         if (key == "jacobi_threads") return jacobi_override ? jacobi_override : jacobi_threads_for(R);
         if (key == "fp32_levels") return fp32_levels;
         if (key == "fp32_leaf") return leaf32;
+        if (key == "leaf_mfma") return leaf_mfma;
         if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
         throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
     }

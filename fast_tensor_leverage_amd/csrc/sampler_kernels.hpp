@@ -103,6 +103,95 @@ __global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a, uint32_t lpb) {
     }
 }
 
+// Leaf Grams of FP32 samplers on the FP64 matrix cores (R <= 16 NB <= 64): a
+// wave per leaf, four leaves per block.  v_mfma_f64_16x16x4f64 takes A
+// (16 x 4) with A[i][k] in lane 16k + i and B (4 x 16) with B[k][j] in lane
+// 16k + j and accumulates D (16 x 16) with D[4v + l/16][l%16] in item v of
+// lane l; with lane l holding x_b = U[row 4s + l/16][16b + l%16] of the leaf,
+// mfma(x_a, x_b) adds rows 4s..4s+3 to Gram block (a, b), a <= b.  The sums
+// are those of the rows in groups of four (their order inside the matrix
+// core), so the FP64 nodes equal k_leaf_gram's to rounding, not bitwise --
+// used only where the bottom levels are stored FP32.  Writes the FP32 factor
+// rows, the FP64 leaf and its FP32 copy like k_leaf_gram.
+typedef double lg_f64x4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ void __launch_bounds__(256) k_leaf_gram_mfma(BuildArgs a) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    const uint32_t R = a.R;
+    const uint32_t F = (uint32_t)a.sh.F;
+    const uint32_t T = (uint32_t)tri_size(R);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.sh.leaf_count) return;  // whole waves; no barrier follows
+    const uint64_t node = a.first_leaf + i;
+    const uint64_t r0 = leaf_index(a.sh, node) * F;
+    const uint32_t rows = (uint32_t)min<uint64_t>(F, a.sh.n - r0);
+    const uint32_t kr = lane >> 4, ci = lane & 15;
+    const double* __restrict__ U = a.U + r0 * R;
+    float* __restrict__ U32 = a.U32 ? a.U32 + r0 * R : nullptr;
+    lg_f64x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = lg_f64x4{0.0, 0.0, 0.0, 0.0};
+    // four 4-row steps per trip: their 4 NB loads are issued before the MFMAs
+    for (uint32_t s16 = 0; s16 < rows; s16 += 16) {
+        double x[4][NB];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t row = s16 + 4 * u + kr;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const uint32_t col = 16 * b + ci;
+                x[u][b] = row < rows && col < R ? U[row * R + col] : 0.0;
+            }
+        }
+        if (U32) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t row = s16 + 4 * u + kr;
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const uint32_t col = 16 * b + ci;
+                    if (row < rows && col < R) U32[row * R + col] = (float)x[u][b];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int t = 0;
+#pragma unroll
+            for (int p = 0; p < NB; p++)
+#pragma unroll
+                for (int q = p; q < NB; q++) {
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][p], x[u][q], acc[t], 0, 0, 0);
+                    t++;
+                }
+        }
+    }
+    double* out = a.tree + node * (uint64_t)a.ts64;
+    int t = 0;
+#pragma unroll
+    for (int p = 0; p < NB; p++)
+#pragma unroll
+        for (int q = p; q < NB; q++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const uint32_t r = 16 * p + 4 * v + kr, c = 16 * q + ci;
+                if (r < R && c < R && r <= c) {
+                    const uint32_t e = r * R - r * (r - 1) / 2 + c - r;
+                    out[e] = acc[t][v];
+                    store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, acc[t][v]);
+                }
+            }
+            t++;
+        }
+    const uint32_t tsw = a.tree32 && a.ts32 > a.ts64 ? a.ts32 : a.ts64;
+    for (uint32_t e = T + lane; e < tsw; e += 64) {
+        if (e < a.ts64) out[e] = 0.0;
+        store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, 0.0);
+    }
+}
+
 // Nodes [start, start + count) of one level: parent = left + right, a block
 // per node.
 __global__ void k_level_sum(double* tree, uint64_t start, uint64_t count, uint32_t ts64, float* tree32,

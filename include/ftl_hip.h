@@ -140,7 +140,10 @@ void* ftl_sampler_stream(ftl_sampler* s);
  * 0: every mode in one launch), "fp32_levels" (FP32 mode:
  * number of bottom tree levels stored FP32; rebuilds the FP32 copies),
  * "fp32_leaf" (FP32 mode: 1, the default, scans the leaf rows of the FP32
- * factor copy; 0 the FP64 factors),
+ * factor copy; 0 the FP64 factors), "leaf_mfma" (FP32 mode, R <= 64: 1, the
+ * default, builds the leaf Grams on the FP64 matrix cores -- equal to the
+ * sequential row sums to rounding; 0: the sequential sums, bitwise those of
+ * an FP64 sampler; applies to later builds),
  * "draw_kernel" (0 = by rank: a group of lanes per sample for R <= 64, one
  * sample per block above; 1 = one sample per wave, R <= 32; 2 = one sample
  * per block, R > 32; 3 = a group of lanes per sample at any rank),

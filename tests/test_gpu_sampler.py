@@ -338,3 +338,31 @@ def test_fp32_copies_written_by_the_build_equal_a_separate_conversion(ftl, R):
     s.KRPDrawSamples_materialize(4, b, hb, wb, draws)
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(wa, wb)
+
+
+@pytest.mark.parametrize("R", [16, 32, 64])
+def test_fp32_sampler_matrix_core_leaf_grams(ftl, oracle, R):
+    """FP32 samplers build their leaf Grams on the matrix cores: the FP64 tree
+    equals the oracle's to rounding; with leaf_mfma = 0 the sequential sums
+    give it bitwise; the draws of the two trees agree on all but a few
+    boundary samples."""
+    print(f"[gpu] matrix-core leaf grams R={R}", flush=True)
+    dims = [5000 + R, 700, 300]
+    U = _factors(dims, R, 300 + R)
+    J = 4000
+    ref = oracle.OracleSampler(U, J, R)
+    s = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    assert s.get_option("leaf_mfma") == 1
+    for k in range(3):
+        t_gpu, t_ref = s.get_tree(k), ref.tree(k)
+        np.testing.assert_allclose(t_gpu, t_ref, rtol=1e-12, atol=1e-13 * np.abs(t_ref).max())
+    draws = np.random.default_rng(R).random(6 * J)
+    a = np.zeros((3, J), dtype=np.uint64)
+    s.KRPDrawSamples(4, a, draws)
+    s.set_option("leaf_mfma", 0)
+    for k in range(3):
+        s.update_sampler(k, U[k])
+        np.testing.assert_array_equal(s.get_tree(k), ref.tree(k))
+    b = np.zeros((3, J), dtype=np.uint64)
+    s.KRPDrawSamples(4, b, draws)
+    assert np.mean(np.any(a != b, axis=0)) < 1e-3
