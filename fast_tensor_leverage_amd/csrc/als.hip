@@ -564,7 +564,9 @@ struct LKSampler {
     uint64_t *cand[kMaxModes] = {}, *ncand = nullptr, *hoff = nullptr, *ndet = nullptr;
     uint32_t *hcnt = nullptr, *hfail = nullptr;
     CubTemp sel;
-    double info[4] = {};  // last draw: deterministic samples, their mass, candidates, 1 if enumeration skipped
+    // last draw: deterministic samples, their mass, candidates, 1 if the
+    // enumeration was skipped, samples whose rejection attempts ran out
+    double info[5] = {};
     uint32_t* counters = nullptr;
     uint64_t seed = 0x243F6A8885A308D3ull, call = 0;
     // staging of the host-pointer API
@@ -776,11 +778,8 @@ struct LKSampler {
             FTL_HIP_CHECK(hipStreamSynchronize(st));
             info[0] = (double)nd;
             info[1] = pd;
+            info[4] = (double)nf;
             FTL_REQUIRE(nd <= J, "hybrid sampler: more than J tuples reach probability 1/J");
-            if (nf)
-                throw Error{FTL_EINVAL, "hybrid sampler: " + std::to_string(nf) + " samples still rejected after " +
-                                            std::to_string(kHybMaxAttempts) +
-                                            " attempts (the deterministic tuples hold almost all the mass)"};
         }
     }
 
@@ -1554,10 +1553,10 @@ int ftl_lkh_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R
     });
 }
 
-int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out4) {
+int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out5) {
     return guarded([&] {
-        FTL_REQUIRE(s && out4, "null argument");
-        for (int i = 0; i < 4; i++) out4[i] = s->impl.info[i];
+        FTL_REQUIRE(s && out5, "null argument");
+        for (int i = 0; i < 5; i++) out5[i] = s->impl.info[i];
     });
 }
 

@@ -1597,7 +1597,10 @@ __global__ void k_hyb_zero(uint64_t* __restrict__ ndet, double* __restrict__ pde
 // Attempt 0 takes the uniform of (i, k) from draws (when given) or the
 // built-in stream at stage 2k; attempt a >= 1 the built-in stream at stage
 // 2k of call (call XOR a 2^40).  A sample still rejected after
-// kHybMaxAttempts attempts keeps its last tuple and counts in *fail.
+// kHybMaxAttempts attempts keeps its last tuple with weight 0 and counts in
+// *fail: the mass below tau it failed to find is then below what that many
+// attempts reach (all of it is deterministic when p_det = 1, where the
+// reference's loop never ends).
 __global__ void k_hyb_reject(LKArgs a, const uint64_t* __restrict__ ndet, const double* __restrict__ pdet,
                              uint32_t* __restrict__ fail) {
 #pragma clang fp contract(off)
@@ -1608,6 +1611,7 @@ __global__ void k_hyb_reject(LKArgs a, const uint64_t* __restrict__ ndet, const 
         if (i < nd) continue;
         double wlog = 0.0;
         uint32_t att = 0;
+        bool exhausted = false;
         for (;; att++) {
             wlog = 0.0;
             for (int k = 0; k < a.N; k++) {
@@ -1625,10 +1629,11 @@ __global__ void k_hyb_reject(LKArgs a, const uint64_t* __restrict__ ndet, const 
             if (exp(wlog) < tau) break;
             if (att + 1 >= kHybMaxAttempts) {
                 atomicAdd(fail, 1u);
+                exhausted = true;
                 break;
             }
         }
-        if (a.w) a.w[i] = scale * exp((0.0 - log((double)a.J)) - wlog);
+        if (a.w) a.w[i] = exhausted ? 0.0 : scale * exp((0.0 - log((double)a.J)) - wlog);
     }
 }
 

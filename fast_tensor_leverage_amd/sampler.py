@@ -370,11 +370,12 @@ class LarsenKoldaSampler:
 
     def last_info(self) -> dict:
         """Hybrid sampler, last draw: deterministic samples, their probability
-        mass, candidate tuples, whether the enumeration was skipped."""
-        out = np.zeros(4)
+        mass, candidate tuples, whether the enumeration was skipped, samples
+        given weight 0 after their rejection attempts ran out."""
+        out = np.zeros(5)
         _lib.check(_lib.load().ftl_lk_sampler_info(self.handle, _ptr(out)), "last_info")
         return {"deterministic": int(out[0]), "p_det": float(out[1]), "candidates": float(out[2]),
-                "skipped": bool(out[3])}
+                "skipped": bool(out[3]), "exhausted": int(out[4])}
 
     def leverage(self, k: int):
         """(leverage scores of factor k, their total)."""

@@ -344,16 +344,20 @@ int ftl_lk_sampler_destroy(ftl_lk_sampler* s);
  * product distribution until p < tau, weight (1 - p_det) / (J p), p_det the
  * kept tuples' mass.  Attempt 0 of sample i takes draws[k J + i] (or the
  * built-in stream at stage 2k), attempt a >= 1 the built-in stream at stage
- * 2k of call (call XOR a 2^40).  Limits (the reference has none): at most
- * 2^32 candidate tuples (the reference skips the deterministic part only from
- * 2^62), 65536 attempts per sample, factor row counts below 2^31; exceeding
- * them fails the draw.  Draws with this handle synchronise its stream. */
+ * 2k of call (call XOR a 2^40).  A sample whose 65536 attempts are all
+ * rejected keeps its last tuple with weight 0 (the reference's loop never
+ * ends, e.g. when the deterministic tuples hold all the mass).  Limits (the
+ * reference has none): at most 2^32 candidate tuples (the reference skips the
+ * deterministic part only from 2^62) and factor row counts below 2^31;
+ * exceeding them fails the draw.  Draws with this handle synchronise its
+ * stream. */
 int ftl_lkh_sampler_create(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U_host,
                            ftl_lk_sampler** out);
 /* Last draw of a hybrid handle: out[0] deterministic samples, out[1] their
  * probability mass, out[2] candidate tuples, out[3] 1 if the deterministic
- * part was skipped (2^62 candidates or more). */
-int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out4);
+ * part was skipped (2^62 candidates or more), out[4] samples given weight 0
+ * after their rejection attempts ran out. */
+int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out5);
 /* KRPDrawSamples / KRPDrawSamples_materialize: samples N x J (rows k != j),
  * h J x R and weights J optional (NULL), draws optional; returns when done. */
 int ftl_lk_sampler_draw(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
