@@ -575,9 +575,18 @@ struct LKSampler {
     Arena mem;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // update_sampler runs on `stream`, draws on the caller's stream: upd_ev
+    // (after an update) orders the next draw, draw_ev (after a draw on
+    // another stream) the next update, so neither reads scores the other is
+    // rewriting
+    hipEvent_t upd_ev = nullptr, draw_ev = nullptr;
+    bool draw_pending = false;
 
     ~LKSampler() {
+        if (draw_pending && draw_ev) (void)hipEventSynchronize(draw_ev);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (upd_ev) (void)hipEventDestroy(upd_ev);
+        if (draw_ev) (void)hipEventDestroy(draw_ev);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
@@ -658,6 +667,10 @@ struct LKSampler {
     // offsets and total, prefix sums
     void update(uint32_t k) {
         FTL_REQUIRE(k < N, "update_sampler: mode index out of range");
+        if (draw_pending) {
+            FTL_HIP_CHECK(hipStreamWaitEvent(stream, draw_ev, 0));
+            draw_pending = false;
+        }
         const uint64_t n = I[k];
         const uint32_t r = (uint32_t)R;
         const unsigned nb = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, (n + 4095) / 4096));
@@ -719,6 +732,7 @@ struct LKSampler {
         FTL_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_cum_rows, dim3(g), dim3(256), 0, stream, lev[k], n, off[k], cum[k]);
         FTL_LAUNCH_CHECK();
+        if (!upd_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&upd_ev, hipEventDisableTiming));
         if (hybrid) {
             // alpha_k = max lev / total (larsen_kolda_hybrid.hpp:101-116)
             const unsigned nbm = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, n / 16384));
@@ -727,6 +741,7 @@ struct LKSampler {
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(64), 0, stream, bmax, nbm, total, k, alpha);
             FTL_LAUNCH_CHECK();
         }
+        FTL_HIP_CHECK(hipEventRecord(upd_ev, stream));
     }
 
     // KRPDrawSamples(j) (larsen_kolda_sampler.hpp:102-133): samples (N x J,
@@ -737,6 +752,7 @@ struct LKSampler {
         FTL_REQUIRE(samples != nullptr, "null samples buffer");
         FTL_REQUIRE(N > 1 || j >= N, "no mode left to sample (N == 1 and j == 0)");
         if (!st) st = stream;
+        if (st != stream && upd_ev) FTL_HIP_CHECK(hipStreamWaitEvent(st, upd_ev, 0));
         LKArgs a{};
         for (uint32_t k = 0; k < N; k++) {
             a.cum[k] = cum[k];
@@ -766,6 +782,11 @@ struct LKSampler {
             FTL_LAUNCH_CHECK();
         }
         if (!draws) call++;
+        if (st != stream) {
+            if (!draw_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&draw_ev, hipEventDisableTiming));
+            FTL_HIP_CHECK(hipEventRecord(draw_ev, st));
+            draw_pending = true;
+        }
         if (hybrid) {
             // the deterministic count, its mass and the rejection failures
             // are checked once the draw is done

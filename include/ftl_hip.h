@@ -362,7 +362,9 @@ int ftl_lk_sampler_info(ftl_lk_sampler* s, double* out5);
  * h J x R and weights J optional (NULL), draws optional; returns when done. */
 int ftl_lk_sampler_draw(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
                         const double* draws);
-/* Same on device buffers, enqueued on stream (NULL: the handle's). */
+/* Same on device buffers, enqueued on stream (NULL: the handle's).  A draw on
+ * another stream waits for the last update_sampler (events, no host wait),
+ * and the next update_sampler waits for the last such draw. */
 int ftl_lk_sampler_draw_device(ftl_lk_sampler* s, uint32_t j, uint64_t* samples, double* h, double* weights,
                                const double* draws, void* stream);
 /* update_sampler(j) (larsen_kolda_sampler.hpp:45-100) from a new factor j

@@ -367,3 +367,34 @@ def test_lk_leverage_both_pinv_routes(ftl, oracle, kind):
     np.testing.assert_allclose(tot, rtot, rtol=1e-8)
     expect_rank = R - 3 if kind == "rank_deficient" else R
     assert abs(tot - expect_rank) < 1e-6 * R
+
+
+def test_lk_device_draws_and_updates_are_ordered(ftl, oracle):
+    """Device draws on a torch stream, then update_sampler with no host
+    synchronisation between them, then another device draw: each draw sees
+    the factors current at its call."""
+    import torch
+    rng = np.random.default_rng(61)
+    dims, R, J = (400000, 3000, 2000), 16, 20000
+    U = [rng.standard_normal((d, R)) for d in dims]
+    U1 = rng.standard_normal((dims[1], R)) * rng.random((dims[1], 1))
+    s = ftl.Sampler(U, J, R, "larsen_kolda")
+    dev = torch.device("cuda:0")
+    draws = rng.random(2 * 3 * J)
+    td = torch.from_numpy(draws).to(dev)
+    st = torch.cuda.current_stream()
+    outs = [torch.zeros((3, J), dtype=torch.int64, device=dev) for _ in range(4)]
+    for o in outs[:3]:
+        s.draw_device(4, o, None, None, td, st)
+    s.update_sampler(1, U1)  # must not overwrite mode 1's scores under the running draws
+    s.draw_device(4, outs[3], None, None, td, st)
+    st.synchronize()
+    before = oracle.OracleLKSampler(U, J, R).draw(4, draws)
+    after = oracle.OracleLKSampler([U[0], U1, U[2]], J, R).draw(4, draws)
+    for o in outs[:3]:
+        got = o.cpu().numpy().view(np.uint64)
+        sure = before["margins"] > 1e-9
+        np.testing.assert_array_equal(got[:, sure], before["samples"][:, sure])
+    got = outs[3].cpu().numpy().view(np.uint64)
+    sure = after["margins"] > 1e-9
+    np.testing.assert_array_equal(got[:, sure], after["samples"][:, sure])
