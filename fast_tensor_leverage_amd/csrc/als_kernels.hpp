@@ -744,12 +744,17 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
 #pragma clang fp contract(off)
     constexpr int GPB = 256 / G;
     constexpr int PINV_LDS = 64 * 64;
+    // one column per lane (CPL == 1): the lane keeps its column of pinv in
+    // registers, so a row's product reads only the row sums from LDS; the
+    // column-loop form stages pinv in LDS (static LDS bounds the blocks a CU
+    // holds, so the register form allocates none for it)
+    constexpr bool kRegP = SOLVE && CPL == 1;
     __shared__ double mt_s[GPB][G * CPL];
-    __shared__ double pinv_s[SOLVE ? PINV_LDS : 1];
+    __shared__ double pinv_s[(SOLVE && !kRegP) ? PINV_LDS : 1];
     __shared__ double sq_s[GPB][G * CPL];
     const uint32_t D = counters[kCntRows], W = counters[kCntPieces];
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
-    const bool pinv_in_lds = SOLVE && R * R <= (uint32_t)PINV_LDS;
+    const bool pinv_in_lds = SOLVE && !kRegP && R * R <= (uint32_t)PINV_LDS;
     if constexpr (SOLVE) {
         if (pinv_in_lds)
             for (uint32_t e = threadIdx.x; e < R * R; e += 256) pinv_s[e] = pinv[e];
@@ -759,21 +764,34 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
 #pragma unroll
     for (int q = 0; q < CPL; q++) sq[q] = 0.0;
     __syncthreads();
-    // one column per lane: the lane keeps its column of pinv in registers
-    // (zero past R), so a row's product reads only the row sums from LDS
-    constexpr bool kRegP = SOLVE && CPL == 1;
-    double pc[kRegP ? G : 1];
+    double pc[kRegP ? G : 1];  // pinv column `lane` (zero past R)
     if constexpr (kRegP) {
 #pragma unroll
         for (int k = 0; k < G; k++) pc[k] = ((uint32_t)k < R && lane < R) ? P[k * R + lane] : 0.0;
     }
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
     // groups run independently: a group stages its row in its own LDS slot,
-    // and a group is at most one wave, whose LDS accesses are ordered
-    for (uint64_t d = (uint64_t)blockIdx.x * GPB + gi; d < D; d += groups) {
+    // and a group is at most one wave, whose LDS accesses are ordered.  The
+    // piece range and row of a group's next segment are fetched while the
+    // current one is summed.
+    uint64_t d = (uint64_t)blockIdx.x * GPB + gi;
+    uint32_t nx0 = 0, nx1 = 0, nxrow = 0;
+    if (d < D) {
+        nx0 = piece_off[d];
+        nx1 = d + 1 < D ? piece_off[d + 1] : W;
+        nxrow = seg_row[d];
+    }
+    for (; d < D; d += groups) {
         double mt[CPL];
-        const uint32_t p0 = piece_off[d];
-        uint32_t p1 = d + 1 < D ? piece_off[d + 1] : W;
+        const uint32_t p0 = nx0;
+        uint32_t p1 = nx1;
+        const uint64_t row = nxrow;
+        const uint64_t dn = d + groups;
+        if (dn < D) {
+            nx0 = piece_off[dn];
+            nx1 = dn + 1 < D ? piece_off[dn + 1] : W;
+            nxrow = seg_row[dn];
+        }
         if (p1 - p0 > kHeavyPieces) p1 = p0 + 1;  // k_seg_heavy_sum left the row's sum in piece p0
 #pragma unroll
         for (int q = 0; q < CPL; q++) {
@@ -789,13 +807,12 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 const uint32_t c = lane + q * G;
-                if (c < R) out[(uint64_t)seg_row[d] * R + c] = mt[q];
+                if (c < R) out[row * R + c] = mt[q];
             }
         } else {
 #pragma unroll
             for (int q = 0; q < CPL; q++) mt_s[gi][lane + q * G] = mt[q];
             wave_sync_lds();
-            const uint64_t row = seg_row[d];
             if constexpr (kRegP) {
                 // the same sequential sum over k < R as below
                 double s = 0.0;
