@@ -905,11 +905,25 @@ __global__ void __launch_bounds__(256) k_colnorm_final(const double* __restrict_
 
 __global__ void k_scale_rows(double* __restrict__ U, const uint32_t* __restrict__ rows, const uint32_t* n_dev,
                              uint64_t n_host, uint32_t R, const double* __restrict__ norms) {
-    const uint64_t n = (n_dev ? (uint64_t)*n_dev : n_host) * R;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = e / R, c = e - i * R;
+    const uint64_t n = n_dev ? (uint64_t)*n_dev : n_host;
+    const uint32_t rpb = blockDim.x / R;  // rows a block takes per step
+    if (rpb == 0) {  // R > blockDim: one entry per thread
+        for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * R;
+             e += (uint64_t)gridDim.x * blockDim.x) {
+            const uint64_t i = e / R, c = e - i * R;
+            const uint64_t r = rows ? (uint64_t)rows[i] : i;
+            U[r * R + c] = U[r * R + c] / norms[c];
+        }
+        return;
+    }
+    // thread t: column t % R of row t / R of the block's batch (no index
+    // division inside the loop)
+    const uint32_t rl = threadIdx.x / R, c = threadIdx.x - rl * R;
+    if (rl >= rpb) return;
+    const double nc = norms[c];
+    for (uint64_t i = (uint64_t)blockIdx.x * rpb + rl; i < n; i += (uint64_t)gridDim.x * rpb) {
         const uint64_t r = rows ? (uint64_t)rows[i] : i;
-        U[r * R + c] = U[r * R + c] / norms[c];
+        U[r * R + c] = U[r * R + c] / nc;
     }
 }
 
