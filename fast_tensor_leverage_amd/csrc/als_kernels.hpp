@@ -1144,8 +1144,9 @@ __global__ void k_pad_square(const double* __restrict__ P, uint32_t R, uint32_t 
 // w of block b takes tile t = 4b + w (64 rows; the grid covers all tiles, one
 // per wave, so the scalar loads of Pp stay in straight-line code instead of
 // being hoisted out of a tile loop into spilled registers), stages it through
-// LDS with coalesced loads (row stride RMAX + 2 doubles, so the row reads are
-// free of bank conflicts), and lane l forms
+// LDS with coalesced loads (16-byte column pairs when R is even; row stride
+// RMAX + 2 doubles, so the row reads are free of bank conflicts), and lane l
+// forms
 // lev = sum_k u_k (P_kk u_k + 2 sum_{c>k} P_kc u_c) from its row in registers
 // and the wave-uniform entries of Pp (RMAX x RMAX, zero outside R x R: padded
 // columns contribute exact zeros).  tsum[t] = the inclusive 64-lane scan's
@@ -1163,16 +1164,33 @@ __global__ void __launch_bounds__(256, 2) k_lev_rows(const double* __restrict__ 
     const uint64_t ntiles = (I + kLevTile - 1) / kLevTile;
     const uint64_t t = (uint64_t)blockIdx.x * 4 + w;
     if (t >= ntiles) return;
-    const int lr = lane / RMAX, lc = lane % RMAX;
     const uint64_t row0 = t * kLevTile;
-    double v[RMAX];
+    if ((R & 1u) == 0 && ((uintptr_t)U & 15u) == 0) {
+        // column pairs, one 16-byte load each (R even and U 16-byte aligned
+        // keep every pair aligned): RMAX / 2 loads per tile
+        constexpr int PPR = RMAX / 2;   // pairs per row
+        constexpr int RPL = 64 / PPR;   // rows per load
+        const int lr = lane / PPR, lp = lane % PPR;
+        double2 v[PPR];
 #pragma unroll
-    for (int q = 0; q < RMAX; q++) {
-        const uint64_t row = row0 + q * RPI + lr;
-        v[q] = (row < I && (uint32_t)lc < R) ? U[row * R + lc] : 0.0;
+        for (int q = 0; q < PPR; q++) {
+            const uint64_t row = row0 + q * RPL + lr;
+            const uint32_t c = 2u * (uint32_t)lp;
+            v[q] = (row < I && c < R) ? *reinterpret_cast<const double2*>(U + row * R + c) : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int q = 0; q < PPR; q++) *reinterpret_cast<double2*>(tile + (q * RPL + lr) * ST + 2 * lp) = v[q];
+    } else {
+        const int lr = lane / RMAX, lc = lane % RMAX;
+        double v[RMAX];
+#pragma unroll
+        for (int q = 0; q < RMAX; q++) {
+            const uint64_t row = row0 + q * RPI + lr;
+            v[q] = (row < I && (uint32_t)lc < R) ? U[row * R + lc] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < RMAX; q++) tile[(q * RPI + lr) * ST + lc] = v[q];
     }
-#pragma unroll
-    for (int q = 0; q < RMAX; q++) tile[(q * RPI + lr) * ST + lc] = v[q];
     wave_sync_lds();
     double u[RMAX];
 #pragma unroll
