@@ -483,18 +483,22 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
     hipLaunchKernelGGL(k_seg_heads, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, m.segid, T, m.seg_start,
                        m.seg_row, m.counters);
     FTL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_seg_pieces, dim3(grid_for(T, 256)), dim3(256), 0, st, m.seg_start, m.counters, T,
+    // bounds known on the host: D <= dcap (the segments are distinct rows of
+    // mode j) and W <= wcap (a segment of n >= 1 matches has ceil(n /
+    // kSegPiece) <= min(n, n / kSegPiece + 1) pieces); the scans below run
+    // over these prefixes, which hold every entry that is read
+    const uint64_t dcap = std::min<uint64_t>(T, X.dims[j]);
+    const uint64_t wcap = std::min<uint64_t>(T, dcap + (T + kSegPiece - 1) / kSegPiece);
+    hipLaunchKernelGGL(k_seg_pieces, dim3(grid_for(dcap, 256)), dim3(256), 0, st, m.seg_start, m.counters, dcap,
                        m.npieces);
     FTL_LAUNCH_CHECK();
-    exclusive_sum(cub, m.npieces, m.piece_off, T, st);
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, T, m.counters,
+    exclusive_sum(cub, m.npieces, m.piece_off, dcap, st);
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, dcap, m.counters,
                        (int)kCntPieces);
     FTL_LAUNCH_CHECK();
-    // segment of every piece (W <= T: a segment of n >= 1 matches has
-    // ceil(n / kSegPiece) <= n pieces)
-    const uint64_t wcap = T;
+    // segment of every piece
     FTL_HIP_CHECK(hipMemsetAsync(m.piece_mark, 0, wcap * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(T, 256, 8192)), dim3(256), 0, st, m.piece_off, m.counters,
+    hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(dcap, 256, 8192)), dim3(256), 0, st, m.piece_off, m.counters,
                        m.piece_mark);
     FTL_LAUNCH_CHECK();
     inclusive_max(cub, m.piece_mark, m.piece_seg, wcap, st);
