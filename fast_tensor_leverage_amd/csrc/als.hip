@@ -473,8 +473,13 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
                         const double* pinv, double* out, double* colsq, Matches& m, Arena& ar, CubTemp& cub,
                         hipStream_t st) {
     m.ensure_t(ar, T, R);
-    hipLaunchKernelGGL(k_expand, dim3(grid_for(T, 256)), dim3(256), 0, st, m.off, m.lo, n, T, X.row[j], m.mrow,
-                       m.mpay);
+    // owner of every match (flags and segid are free until the row sort)
+    FTL_HIP_CHECK(hipMemsetAsync(m.flags, 0, T * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_mark_owners, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, m.off, m.cnt, n, m.flags);
+    FTL_LAUNCH_CHECK();
+    inclusive_max(cub, m.flags, m.segid, T, st);
+    hipLaunchKernelGGL(k_expand, dim3(grid_for(T, 256)), dim3(256), 0, st, m.segid, m.off, m.lo, T, X.row[j],
+                       m.mrow, m.mpay);
     FTL_LAUNCH_CHECK();
     sort_pairs(cub, m.mrow, m.srow, m.mpay, m.spay, T, X.bits[j], st);
     hipLaunchKernelGGL(k_head_flags<uint32_t>, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, T, m.flags);

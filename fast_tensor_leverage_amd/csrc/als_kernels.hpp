@@ -486,19 +486,24 @@ __global__ void k_scan_total(const uint32_t* __restrict__ off, const uint32_t* _
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[slot] = off[n - 1] + cnt[n - 1];
 }
 
-// Match m (sample u's (m - off[u])-th nonzero of its range): output row and
-// payload (u << 32 | sorted position).  Matches are numbered in (sample,
-// position) order.
-__global__ void k_expand(const uint32_t* __restrict__ off, const uint32_t* __restrict__ lo, uint64_t n,
-                         uint64_t T, const uint32_t* __restrict__ rowj, uint32_t* __restrict__ mrow,
-                         uint64_t* __restrict__ mpay) {
+// mark[off[u]] = u for every sample u with matches, on a zeroed array of T
+// entries: its inclusive max-scan gives every match the sample that owns it
+// (the one u with off[u] <= m < off[u] + cnt[u]; samples without matches
+// share their offset with the next sample that has some)
+__global__ void k_mark_owners(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint64_t n,
+                              uint32_t* __restrict__ mark) {
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
+        if (cnt[u] > 0) mark[off[u]] = (uint32_t)u;
+}
+
+// Match m (sample u = owner[m], the (m - off[u])-th nonzero of its range):
+// output row and payload (u << 32 | sorted position).  Matches are numbered
+// in (sample, position) order.
+__global__ void k_expand(const uint32_t* __restrict__ owner, const uint32_t* __restrict__ off,
+                         const uint32_t* __restrict__ lo, uint64_t T, const uint32_t* __restrict__ rowj,
+                         uint32_t* __restrict__ mrow, uint64_t* __restrict__ mpay) {
     for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < T; m += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t l = 0, h = n;  // largest u with off[u] <= m
-        while (h - l > 1) {
-            const uint64_t mid = (l + h) >> 1;
-            if (off[mid] <= m) l = mid;
-            else h = mid;
-        }
+        const uint64_t l = owner[m];
         const uint64_t pos = (uint64_t)lo[l] + (m - off[l]);
         mrow[m] = rowj[pos];
         mpay[m] = (l << 32) | pos;
