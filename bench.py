@@ -33,20 +33,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(I, R, J, N, fp64_levels, fp32):
-    """Bytes one draw-kernel launch must read: per sample and mode, the
-    left-child Gram of every level below the root (packed upper triangle,
-    FP64 for the top levels, FP32 below in FP32 mode), the F x R leaf block
-    and the R-entry row of U_k used for the h update.  The root and the eigen
-    trees (read by every sample, cache resident) are excluded."""
+def algorithmic_bytes(I, R, J, N, s_G, s_U):
+    """Algorithmic HBM bytes of one draw (all N modes sampled): per sample and
+    mode, the packed upper triangle (T = R(R+1)/2 entries of s_G bytes) of the
+    left child read at each of the L levels below the root, the F x R leaf
+    block and the R-entry factor row of the h update (s_U bytes per entry):
+    B = sum_k J [L_k T s_G + F R s_U + R s_U].  The root Gram and the eigen
+    trees (read by every sample) are excluded.  s_G = s_U = 4: 6.336 GB at
+    N=3, I=2^23, R=32, J=50,000; s_G = s_U = 8: 12.672 GB."""
     F = min(R, I)
     leaves = -(-I // F)
     depth = leaves.bit_length() - 1 + (0 if leaves & (leaves - 1) == 0 else 1)
     T = R * (R + 1) // 2
-    per = 0
-    for d in range(1, depth + 1):
-        per += T * (8 if (not fp32 or d < fp64_levels) else 4)
-    per += F * R * (4 if fp32 else 8) + R * 8
+    per = depth * T * s_G + F * R * s_U + R * s_U
     return per * J * N
 
 
@@ -61,9 +60,11 @@ def lscpu_model():
     return "unknown"
 
 
-def cpu_baseline(U, draws, J, j, gpu_samples, sample_j):
+def cpu_baseline(U, draws, J, j, gpu_samples, sample_j, trials=7):
     """Oracle (FP64 OpenMP restatement) on J' = sample_j samples of the same
-    workload; returns the baseline record and the index comparison."""
+    workload, trees prebuilt: 1 warm-up and `trials` timed draws; the value is
+    J' over the median time (the minimum is reported beside it).  Returns
+    the baseline record and the index comparison with the GPU FP64 sampler."""
     sys.path.insert(0, HERE)
     from oracle import oracle as orc
     orc.build()
@@ -72,20 +73,23 @@ def cpu_baseline(U, draws, J, j, gpu_samples, sample_j):
     ref = orc.OracleSampler(U, sample_j, U[0].shape[1])
     t_build = time.perf_counter() - t0
     sub = np.concatenate([draws[s * J: s * J + sample_j] for s in range(2 * N)])
-    ref.draw(j, sub, want_margins=True)  # warm-up
+    ref.draw(j, sub, want_margins=False)  # warm-up
     times = []
-    for _ in range(2):
+    for _ in range(trials):
         t0 = time.perf_counter()
-        out = ref.draw(j, sub, want_margins=True)
+        ref.draw(j, sub, want_margins=False)
         times.append(time.perf_counter() - t0)
-    t = min(times)
+    out = ref.draw(j, sub, want_margins=True)
+    t_med, t_min = float(np.median(times)), float(min(times))
     diff = np.zeros(sample_j, dtype=bool)
     for k in range(N):
         diff |= gpu_samples[k, :sample_j] != out["samples"][k]
     sure = out["margins"] > 1e-10
-    rec = {"value": sample_j / t, "unit": "samples/s", "cores": orc.num_threads(), "kind": "port",
-           "sample": f"KRPDrawSamples over J'={sample_j} of the same N=3, I=2^23, R=32 workload "
-                     f"(trees prebuilt, {t_build:.1f} s), best of 2 after 1 warm-up; host {lscpu_model()}"}
+    rec = {"value": sample_j / t_med, "unit": "samples/s", "cores": orc.num_threads(), "kind": "port",
+           "value_best": sample_j / t_min, "trial_s": times,
+           "sample": f"KRPDrawSamples over J'={sample_j} samples of the same N=3, I=2^23, R=32 workload "
+                     f"(trees prebuilt in {t_build:.1f} s, not timed); median of {trials} timed draws after "
+                     f"1 warm-up; host {lscpu_model()}"}
     cmp = {"oracle_vs_gpu_fp64_mismatches": int(diff.sum()),
            "oracle_vs_gpu_fp64_mismatches_outside_margin": int((diff & sure).sum()),
            "oracle_compared_samples": sample_j}
@@ -103,46 +107,6 @@ ALS_SIZES = {
 }
 
 
-def zipf_coords(torch, n, size, s, gen, dev):
-    """Truncated Zipf(s) indices in [0, n) by inverse transform of the
-    continuous density x^-s on [1, n+1)."""
-    u = torch.rand(size, generator=gen, device=dev, dtype=torch.float64)
-    a = 1.0 - (n + 1.0) ** (1.0 - s)
-    x = (1.0 - u * a) ** (1.0 / (1.0 - s))
-    return (torch.floor(x).to(torch.int64) - 1).clamp_(0, n - 1)
-
-
-def make_sparse_c5(torch, dims, nnz, R, dev, seed=2024):
-    """Distinct Zipf(1.1) coordinates (int32), values = rank-R CP model with
-    N(0,1) factors + 1% Gaussian noise (FP64), generated on the device."""
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(seed)
-    b1, b2 = (dims[1] - 1).bit_length(), (dims[2] - 1).bit_length()
-    keys = torch.empty(0, dtype=torch.int64, device=dev)
-    while keys.numel() < nnz:
-        m = max(int(1.25 * (nnz - keys.numel())), 1_000_000)
-        k = (zipf_coords(torch, dims[0], m, 1.1, gen, dev) << (b1 + b2)) | \
-            (zipf_coords(torch, dims[1], m, 1.1, gen, dev) << b2) | zipf_coords(torch, dims[2], m, 1.1, gen, dev)
-        keys = torch.unique(torch.cat([keys, k]))
-        del k
-    keys = keys[torch.randperm(keys.numel(), generator=gen, device=dev)[:nnz]]
-    idx = torch.stack([keys >> (b1 + b2), (keys >> b2) & ((1 << b1) - 1), keys & ((1 << b2) - 1)], dim=1)
-    del keys
-    idx = idx.to(torch.int32).contiguous()
-    A = [torch.randn((d, R), generator=gen, device=dev, dtype=torch.float64) for d in dims]
-    vals = torch.empty(nnz, dtype=torch.float64, device=dev)
-    step = 10_000_000
-    for s0 in range(0, nnz, step):
-        s1 = min(nnz, s0 + step)
-        p = A[0][idx[s0:s1, 0].long()]
-        for k in range(1, 3):
-            p *= A[k][idx[s0:s1, k].long()]
-        vals[s0:s1] = p.sum(dim=1)
-    vals += 0.01 * vals.std() * torch.randn(nnz, generator=gen, device=dev, dtype=torch.float64)
-    del A
-    return idx, vals
-
-
 def als_main(args):
     import ctypes
     import torch
@@ -156,7 +120,8 @@ def als_main(args):
     N = len(dims)
     log(f"ALS workload {args.als_size}: dims={dims} nnz={nnz} R={R} J={J} sweeps={sweeps}")
     t0 = time.perf_counter()
-    idx, vals = make_sparse_c5(torch, dims, nnz, R, dev)
+    from fast_tensor_leverage_amd.synth import zipf_sparse
+    idx, vals = zipf_sparse(torch, dims, nnz, R, dev)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     log(f"generated {nnz} nonzeros in {t_gen:.1f} s")
@@ -347,15 +312,8 @@ def c3_main(args):
     rows = []
     for R in [int(r) for r in args.c3_ranks.split(",")]:
         log(f"C3 R={R}: building ill-conditioned factors")
-        gen = torch.Generator(device=dev)
-        gen.manual_seed(3000 + R)
-        s = torch.logspace(0, -6, R, dtype=torch.float64, device=dev)
-        U = []
-        for k in range(N):
-            qu, _ = torch.linalg.qr(torch.randn((I, R), generator=gen, device=dev, dtype=torch.float64))
-            qv, _ = torch.linalg.qr(torch.randn((R, R), generator=gen, device=dev, dtype=torch.float64))
-            U.append(((qu * s) @ qv.T).contiguous())
-            del qu, qv
+        from fast_tensor_leverage_amd.synth import ill_conditioned_factors
+        U, gen = ill_conditioned_factors(torch, N, I, R, dev, seed=3000 + R)
         draws = torch.rand(2 * N * J, generator=gen, device=dev, dtype=torch.float64)
         res = {}
         for prec in ("fp64", "fp32"):
@@ -372,13 +330,31 @@ def c3_main(args):
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             res[prec] = {"samples_per_s": args.c3_steps * J / el, "kernel_ms": tm,
-                         "samples": d_s.cpu().numpy().copy(), "p": (1.0 / (J * d_w)).cpu().numpy()}
+                         "samples": d_s.cpu().numpy().copy(), "p": (1.0 / (J * d_w)).cpu().numpy(),
+                         "fp32_levels": smp.get_option("fp32_levels") if prec == "fp32" else None}
             log(f"C3 R={R} {prec}: {res[prec]['samples_per_s']:,.0f} samples/s, draw {tm['draw_ms']:.3f} ms")
             smp.close()
             del smp, d_s, d_w
             torch.cuda.synchronize()
         diff = np.any(res["fp32"]["samples"] != res["fp64"]["samples"], axis=0)
-        p32, p64 = res["fp32"]["p"], res["fp64"]["p"]
+        # the FP32 path's probability of each row it drew, 1 / (J w), against
+        # the FP64 probability of the same row, h^T pinv(G) h / R (h the FP64
+        # product of the drawn rows, pinv with the 1e-8 cut of the Hadamard
+        # product of the FP64 Grams)
+        G = torch.ones((R, R), dtype=torch.float64, device=dev)
+        for M in U:
+            G *= M.T @ M
+        lam, V = torch.linalg.eigh(G)
+        inv = torch.where(lam > 1e-8, 1.0 / torch.where(lam > 1e-8, lam, torch.ones_like(lam)),
+                          torch.zeros_like(lam))
+        pinv = (V * inv) @ V.T
+        rows32 = torch.from_numpy(res["fp32"]["samples"]).to(dev)
+        hh = torch.ones((J, R), dtype=torch.float64, device=dev)
+        for k in range(N):
+            hh *= U[k][rows32[k]]
+        p64 = (torch.einsum("ir,rs,is->i", hh, pinv, hh) / R).cpu().numpy()
+        p32 = res["fp32"]["p"]
+        del hh, rows32
         # rows whose leverage under the cut pinv is zero (hᵀ pinv h = 0: the
         # row lies in the discarded eigenspace) have no relative error
         ok = (p64 > 0) & np.isfinite(p64) & np.isfinite(p32)
@@ -390,6 +366,7 @@ def c3_main(args):
                                  "fp32": res["fp32"]["kernel_ms"]["draw_ms"]},
                      "computeM_ms": res["fp64"]["kernel_ms"]["gram_pinv_ms"] + res["fp64"]["kernel_ms"]["eigen_modes_ms"],
                      "index_mismatch_rate": float(diff.mean()),
+                     "fp32_levels": res["fp32"]["fp32_levels"],
                      "leverage_prob_rel_diff_max": float(rel.max()) if rel.size else None,
                      "leverage_prob_rel_diff_mean": float(rel.mean()) if rel.size else None,
                      "drawn_rows_with_zero_leverage_fp64": float(1.0 - ok.mean())})
@@ -548,7 +525,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--cpu-sample", type=int, default=J_SAMPLES)
+    ap.add_argument("--cpu-trials", type=int, default=7)
     ap.add_argument("--I", type=int, default=I_ROWS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (reference kernel for the "
@@ -637,17 +615,26 @@ def main():
     v64 = args.steps * J / f64["elapsed"]
     mism = float(np.mean(np.any(samples_out["fp32"] != samples_out["fp64"], axis=0)))
     fp64_levels = f32["info"]["fp64_levels"]
-    bytes32 = algorithmic_bytes(I, R, J, N, fp64_levels, True)
-    bytes64 = algorithmic_bytes(I, R, J, N, 0, False)
+    # algorithmic bytes with every level at the stored width the algorithm
+    # needs (FP32 nodes and rows; FP64 for the plain sampler)
+    bytes32 = algorithmic_bytes(I, R, J, N, 4, 4)
+    bytes64 = algorithmic_bytes(I, R, J, N, 8, 8)
     t_kernel = f32["timing"]["draw_ms"] * 1e-3
     achieved = bytes32 / t_kernel
+    if achieved > HBM_PEAK:
+        raise SystemExit(f"roofline check failed: {bytes32 / 1e9:.3f} GB in {t_kernel * 1e3:.3f} ms exceeds the "
+                         f"HBM peak; the byte count or the kernel time is wrong")
     # HBM bytes of the FP32 draw (its per-mode launches together), from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)
     traffic = None
+    traffic64 = None
     tfile = os.path.join(HERE, "profiles", "k_krp_draw_N3_I2p23_R32_J50000_traffic.json")
     if os.path.exists(tfile) and I == I_ROWS:
         with open(tfile) as f:
-            traffic = json.load(f)["fp32"]["hbm_bytes_per_draw"]
+            tj = json.load(f)
+        traffic = tj["fp32"]["hbm_bytes_per_draw"]
+        traffic64 = tj["fp64"]["hbm_bytes_per_draw"]
+    t64 = f64["timing"]["draw_ms"] * 1e-3
 
     serving = None
     if not args.skip_serve:
@@ -659,7 +646,8 @@ def main():
     extra_cmp = {}
     if not args.skip_cpu:
         log("CPU baseline (oracle)")
-        cpu, extra_cmp = cpu_baseline(U, draws, J, j, samples_out["fp64"], min(args.cpu_sample, J))
+        cpu, extra_cmp = cpu_baseline(U, draws, J, j, samples_out["fp64"], min(args.cpu_sample, J),
+                                      args.cpu_trials)
 
     line = {
         "metric": "KRP samples/s (N=3,I=2^23,R=32) FP32-MFMA vs FP64 GPU; p99 req latency",
@@ -677,17 +665,25 @@ def main():
         "config": {"workload": "KRPDrawSamples N=3 I=2^23 R=32 J=50000 j=N+1 (all modes)",
                    "N": N, "I": I, "R": R, "J": J, "j": j},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic / N if traffic else None,
+                     "traffic_source": "profiles/k_krp_draw_N3_I2p23_R32_J50000_traffic.json (rocprofv3 --pmc "
+                                       "FETCH_SIZE and WRITE_SIZE per draw launch, copy-calibrated), not this run",
                      "traffic_GBps": (traffic / t_kernel / 1e9) if traffic else None,
                      "kernel": f"k_krp_draw<32,{f32['group_lanes']},256,fp32>",
-                     "kernel_ms": f32["timing"]["draw_ms"],
-                     "algorithmic_bytes_per_draw": bytes32,
-                     "launches_per_draw": N},
+                     "kernel_ms_per_launch": f32["timing"]["draw_ms"] / N,
+                     "algorithmic_bytes_per_launch": bytes32 / N,
+                     "algorithmic_bytes_per_sample_mode": bytes32 / (N * J),
+                     "launches_per_draw": N,
+                     "stored_fp64_top_levels": fp64_levels},
         "cpu_baseline": cpu,
         "fp64_gpu_samples_per_s": v64,
         "fp32_over_fp64": value / v64,
         "fp32_vs_fp64_index_mismatch_rate": mism,
-        "fp64_roofline_frac": bytes64 / (f64["timing"]["draw_ms"] * 1e-3) / HBM_PEAK,
+        # the plain sampler's algorithmic bytes exceed what HBM could deliver
+        # in its kernel time (its top levels are served from L2 / MALL), so
+        # its fraction is taken over the counter-measured HBM traffic
+        "fp64_algorithmic_GBps": bytes64 / t64 / 1e9,
+        "fp64_traffic_frac": (traffic64 / t64 / HBM_PEAK) if traffic64 else None,
         "kernel_ms": {"fp32": f32["timing"], "fp64": f64["timing"]},
         "construction_s": {"fp32": f32["build_s"], "fp64": f64["build_s"]},
         "p99_latency_ms": serving["p99_ms"] if serving else None,

@@ -23,6 +23,7 @@ SIGNATURES = {
     "ftl_last_error": ([], ctypes.c_char_p),
     "ftl_version": ([], ctypes.c_char_p),
     "ftl_device_count": ([], _i32),
+    "ftl_eigensolver_capped": ([_vp], _i32),
     "ftl_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_create_device": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_destroy": ([_vp], _i32),
@@ -138,6 +139,14 @@ def check(rc: int, what: str = ""):
     if rc == FTL_EUNSUPPORTED:
         raise NotImplementedError(msg)
     raise FTLError(msg)
+
+
+def eigensolver_capped() -> int:
+    """Eigensolves on the current device that stopped at the Jacobi sweep cap
+    before converging, since the library was loaded (waits for the device)."""
+    out = ctypes.c_uint64()
+    check(load().ftl_eigensolver_capped(ctypes.byref(out)), "eigensolver_capped")
+    return int(out.value)
 
 
 def device_count() -> int:

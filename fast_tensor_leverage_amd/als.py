@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .sampler import PRECISIONS, _ptr, _require_array
+from .sampler import PRECISIONS, _dev_ptr, _ptr, _require_array
 
 
 def _handle_out():
@@ -68,6 +68,37 @@ class SparseTensor:
         self.dims = [int(d) for d in info[2:]]
         self.last_found = 0
 
+    @classmethod
+    def from_device(cls, idx, values, dims=None, method: str = "sort"):
+        """The same from device tensors (torch, on the GPU, contiguous):
+        idx nnz x N int32/uint32 coordinates, values nnz float64.  They are
+        read during the call and not retained."""
+        if method not in ("sort", "hash"):
+            raise ValueError(f"unknown lookup method '{method}' (expected 'sort' or 'hash')")
+        if idx.dim() != 2 or values.dim() != 1 or values.shape[0] != idx.shape[0]:
+            raise ValueError("idx must be nnz x N and values nnz")
+        if str(idx.dtype) not in ("torch.int32", "torch.uint32") or str(values.dtype) != "torch.float64":
+            raise TypeError("idx must be int32/uint32 and values float64")
+        nnz, N = int(idx.shape[0]), int(idx.shape[1])
+        if nnz == 0:
+            raise ValueError("the tensor needs at least one nonzero")
+        _lib.require_device()
+        obj = cls.__new__(cls)
+        obj._h = None
+        dims_arr = None if dims is None else np.ascontiguousarray(dims, dtype=np.uint64)
+        if dims_arr is not None and dims_arr.shape != (N,):
+            raise ValueError("dims must hold one size per mode")
+        h = _handle_out()
+        _lib.check(_lib.load().ftl_sparse_create_device(N, nnz, _dev_ptr(idx, "idx"), _dev_ptr(values, "values"),
+                                                        None if dims_arr is None else _ptr(dims_arr),
+                                                        ctypes.byref(h)), "SparseTensor")
+        obj._h = h
+        info = np.zeros(2 + N, dtype=np.uint64)
+        _lib.check(_lib.load().ftl_sparse_info(obj._h, _ptr(info)))
+        obj.N, obj.nnz, obj.dims, obj.last_found = N, nnz, [int(d) for d in info[2:]], 0
+        obj.method = method
+        return obj
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -100,15 +131,18 @@ class SparseTensor:
                                                  ctypes.byref(found)), "execute_downsampled_mttkrp")
         self.last_found = int(found.value)
 
-    def lookup(self, samples, j: int):
+    def lookup(self, samples, j: int, counts_only: bool = False):
         """Matched nonzeros of every sample on mode j's lookup: (counts per
-        sample, input positions of the matches in sample order)."""
+        sample, input positions of the matches in sample order; None with
+        ``counts_only``)."""
         self._check_samples(samples, j)
         Jp = samples.shape[0]
         counts = np.zeros(Jp, dtype=np.uint64)
         found = ctypes.c_uint64()
         L = _lib.load()
         _lib.check(L.ftl_sparse_lookup(self._h, j, Jp, _ptr(samples), _ptr(counts), None, 0, ctypes.byref(found)))
+        if counts_only:
+            return counts, None
         ids = np.zeros(max(int(found.value), 1), dtype=np.uint64)
         _lib.check(L.ftl_sparse_lookup(self._h, j, Jp, _ptr(samples), None, _ptr(ids), ids.size,
                                        ctypes.byref(found)))

@@ -1312,6 +1312,10 @@ struct ftl_als {
 
 using ftl::guarded;
 
+namespace ftl {
+uint64_t jacobi_capped_als_tu() { return jacobi_capped_read(); }
+}  // namespace ftl
+
 extern "C" {
 
 static int sparse_create_common(uint32_t N, uint64_t nnz, const uint32_t* idx, const double* values,
@@ -1369,10 +1373,18 @@ static uint64_t probe_host_samples(ftl::SparseDev& S, uint32_t j, uint64_t Jp, c
     hipLaunchKernelGGL(k_tuples_u32, dim3(grid_for(Jp * S.N, 256)), dim3(256), 0, st, stage, Jp * S.N, tup);
     FTL_HIP_CHECK(hipGetLastError());
     probe_and_count(S, j, tup, nullptr, Jp, Jp, S.mt, S.mem, S.cub, st);
+    hipLaunchKernelGGL(k_total_hi, dim3(1), dim3(1024), 0, st, S.mt.cnt, Jp, S.mt.counters, (int)kCntMatchesHi);
+    FTL_HIP_CHECK(hipGetLastError());
     FTL_HIP_CHECK(hipMemcpyAsync(S.mt.host_counters, S.mt.counters, kCntSlots * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, st));
     FTL_HIP_CHECK(hipStreamSynchronize(st));
     S.mem.release(stage);
+    // the per-sample offsets and the match buffers are indexed in 32 bits
+    if (S.mt.host_counters[kCntMatchesHi] != 0)
+        throw Error{FTL_EUNSUPPORTED, "the samples match 2^32 or more nonzeros in total (" +
+                                          std::to_string(((uint64_t)S.mt.host_counters[kCntMatchesHi] << 32) |
+                                                         S.mt.host_counters[kCntMatches]) +
+                                          "); pass fewer or deduplicated samples"};
     return S.mt.host_counters[kCntMatches];
 }
 

@@ -66,7 +66,15 @@ struct FactorPtrs {
 // Counters the ALS pipeline keeps on the device: unique samples J', matched
 // nonzeros T, touched rows D, reduction pieces W, heavy rows (segments of more
 // than kHeavyPieces pieces).
-enum { kCntUnique = 0, kCntMatches = 1, kCntRows = 2, kCntPieces = 3, kCntHeavy = 4, kCntSlots = 5 };
+enum {
+    kCntUnique = 0,
+    kCntMatches = 1,
+    kCntRows = 2,
+    kCntPieces = 3,
+    kCntHeavy = 4,
+    kCntMatchesHi = 5,  // bits 32..63 of the 64-bit match total (host samples)
+    kCntSlots = 6
+};
 
 __device__ inline uint64_t pack_tuple(const KeyLayout& L, const uint32_t* t) {
     uint64_t key = 0;
@@ -484,6 +492,24 @@ __global__ void k_probe(ProbeArgs a) {
 __global__ void k_scan_total(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint64_t n,
                              uint32_t* __restrict__ counters, int slot) {
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[slot] = off[n - 1] + cnt[n - 1];
+}
+
+// Bits 32..63 of sum(cnt[0..n)) into counters[slot]: the uint32 exclusive
+// scan of cnt wraps once the match total reaches 2^32, which only a host
+// sample list with repeated tuples can reach (a deduplicated one matches
+// disjoint ranges, so at most nnz < 2^32).  One block; the sum is exact.
+__global__ void __launch_bounds__(1024) k_total_hi(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                   uint32_t* __restrict__ counters, int slot) {
+    __shared__ uint64_t part[1024];
+    uint64_t acc = 0;
+    for (uint64_t u = threadIdx.x; u < n; u += 1024) acc += cnt[u];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counters[slot] = (uint32_t)(part[0] >> 32);
 }
 
 // mark[off[u]] = u for every sample u with matches, on a zeroed array of T
@@ -1157,8 +1183,10 @@ __global__ void k_pad_square(const double* __restrict__ P, uint32_t R, uint32_t 
 // columns contribute exact zeros).  tsum[t] = the inclusive 64-lane scan's
 // last value over tile t (rows past I count 0), the same scan k_cum_rows
 // takes, so the tile offsets and the prefix sums agree bitwise.
+// Two blocks per CU up to RMAX = 32; the 64-column tile of RMAX = 64 needs
+// the registers of one.
 template <int RMAX>
-__global__ void __launch_bounds__(256, 2) k_lev_rows(const double* __restrict__ U, uint64_t I, uint32_t R,
+__global__ void __launch_bounds__(256, (RMAX <= 32 ? 2 : 1)) k_lev_rows(const double* __restrict__ U, uint64_t I, uint32_t R,
                                                      const double* __restrict__ Pp, double* __restrict__ lev,
                                                      double* __restrict__ tsum) {
     constexpr int ST = RMAX + 2;

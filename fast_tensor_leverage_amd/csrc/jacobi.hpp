@@ -81,8 +81,26 @@ __device__ inline bool jacobi_rotation(const double* A, int LD, int R, int p, in
     return true;
 }
 
+// Returns false (in every thread) when the sweep cap ended the iteration
+// before every pair met the skip test.
+// Sweep cap of the cyclic Jacobi iteration (quadratic convergence takes
+// 6-10 sweeps for R <= 128 in FP64).
+constexpr int kJacobiMaxSweeps = 40;
+
+// Eigensolves of this translation unit's kernels that stopped at the sweep
+// cap (one device counter per source file and device; read on the host with
+// jacobi_capped_read).
+static __device__ uint32_t g_jacobi_capped = 0;
+
+static inline uint64_t jacobi_capped_read() {
+    uint32_t v = 0;
+    FTL_HIP_CHECK(hipDeviceSynchronize());
+    FTL_HIP_CHECK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_jacobi_capped), sizeof(v), 0, hipMemcpyDeviceToHost));
+    return v;
+}
+
 template <int BS>
-__device__ void jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
+__device__ bool jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
                              double* diag_tmp) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
@@ -91,7 +109,8 @@ __device__ void jacobi_block(double* A, double* V, int R, double* lam, double* V
     const int npairs = n2 / 2;
     for (int e = tid; e < R * R; e += BS) V[(e / R) * LD + e % R] = (e / R == e % R) ? 1.0 : 0.0;
     __syncthreads();
-    for (int sweep = 0; sweep < 40; sweep++) {
+    bool converged = false;
+    for (int sweep = 0; sweep < kJacobiMaxSweeps; sweep++) {
         int any = 0;
         for (int round = 0; round + 1 < n2; round++) {
             // rotations of this round, each computed once from the current A
@@ -156,7 +175,10 @@ __device__ void jacobi_block(double* A, double* V, int R, double* lam, double* V
             }
             __syncthreads();
         }
-        if (!__syncthreads_or(any)) break;
+        if (!__syncthreads_or(any)) {
+            converged = true;
+            break;
+        }
         // converged when every pair meets the skip test: the next sweep would
         // rotate nothing, so stopping here gives the same result
         int open = 0;
@@ -168,7 +190,10 @@ __device__ void jacobi_block(double* A, double* V, int R, double* lam, double* V
                 open |= !(apq * apq <= thr || apq == 0.0);
             }
         }
-        if (!__syncthreads_or(open)) break;
+        if (!__syncthreads_or(open)) {
+            converged = true;
+            break;
+        }
     }
     // stable ascending sort of the diagonal; Vout (R x R, dense) column v is
     // the eigenvector of lam[v]
@@ -185,7 +210,9 @@ __device__ void jacobi_block(double* A, double* V, int R, double* lam, double* V
         if (i == 0) lam[rank] = d;
         Vout[i * R + rank] = V[i * LD + v];
     }
+    if (!converged && tid == 0) atomicAdd(&g_jacobi_capped, 1u);
     __syncthreads();
+    return converged;
 }
 
 // Jacobi storage of a block: A ((R+1) R doubles, rows padded to an odd

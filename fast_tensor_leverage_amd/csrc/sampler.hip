@@ -81,6 +81,19 @@ struct Sampler {
     hipStream_t cm_stream = nullptr;
     hipEvent_t trees_ev = nullptr;
     bool trees_dirty = true;
+    // Draws enqueued on a caller's stream (not the handle's): handle_ev,
+    // recorded on the handle's stream before such a draw, orders it after the
+    // factor copies and tree builds queued there; foreign_ev, recorded on the
+    // caller's stream after it, makes the next factor copy or tree build wait
+    // until that draw has read the old factors and trees.
+    hipEvent_t handle_ev = nullptr, foreign_ev = nullptr;
+    bool foreign_pending = false;
+
+    void wait_foreign_draws() {
+        if (!foreign_pending) return;
+        FTL_HIP_CHECK(hipStreamWaitEvent(stream, foreign_ev, 0));
+        foreign_pending = false;
+    }
     // option "overlap_compute_m" (only with a stream of the sampler's own)
     uint32_t overlap_cm = 1;
     // set 0 (compute_m_host and the single-set samplers read these)
@@ -187,6 +200,9 @@ struct Sampler {
     }
 
     ~Sampler() {
+        if (foreign_pending) (void)hipEventSynchronize(foreign_ev);
+        if (handle_ev) (void)hipEventDestroy(handle_ev);
+        if (foreign_ev) (void)hipEventDestroy(foreign_ev);
         if (cm_stream) (void)hipStreamSynchronize(cm_stream);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& ms : msets) {
@@ -221,6 +237,7 @@ struct Sampler {
     // on a borrowed stream, which keep one computeM set on that stream.
     void build_mode(uint32_t k, hipStream_t st = nullptr) {
         if (!st) st = stream;
+        if (st == stream) wait_foreign_draws();
         trees_dirty = true;
         ModeHost& m = modes[k];
         BuildArgs ba;
@@ -296,6 +313,7 @@ struct Sampler {
         }
         set_mode_precision(m);
         if (precision != FTL_FP32 || m.sh.node_count <= m.n64) return;
+        wait_foreign_draws();
         const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
         m.tree32_elems = n32 + tree_pad;
         m.tree32 = dalloc<float>(m.tree32_elems);
@@ -524,6 +542,7 @@ struct Sampler {
         FTL_REQUIRE(j < N, "update_sampler: mode index out of range");
         FTL_REQUIRE(Uj != nullptr, "update_sampler: null factor");
         ModeHost& m = modes[j];
+        wait_foreign_draws();
         FTL_HIP_CHECK(hipMemcpyAsync(m.U64, Uj, m.I * R * sizeof(double),
                                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
         build_mode(j);
@@ -720,6 +739,13 @@ struct Sampler {
         if (!st) st = stream;
         std::vector<int> order = sampled_modes(j);
         FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
+        const bool foreign = st != stream;
+        if (foreign) {
+            if (!handle_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&handle_ev, hipEventDisableTiming));
+            if (!foreign_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&foreign_ev, hipEventDisableTiming));
+            FTL_HIP_CHECK(hipEventRecord(handle_ev, stream));
+            FTL_HIP_CHECK(hipStreamWaitEvent(st, handle_ev, 0));
+        }
         std::array<hipEvent_t, 5> evs{};
         if (timing)
             for (auto& e : evs) e = take_event();
@@ -793,6 +819,10 @@ struct Sampler {
             dispatch_draw(a, st);
         }
         if (overlap) FTL_HIP_CHECK(hipEventRecord(ms.draw_done, st));
+        if (foreign) {
+            FTL_HIP_CHECK(hipEventRecord(foreign_ev, st));
+            foreign_pending = true;
+        }
         if (timing) {
             FTL_HIP_CHECK(hipEventRecord(evs[4], st));
             pending.push_back(evs);
@@ -914,6 +944,13 @@ int ftl_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int ftl_eigensolver_capped(uint64_t* out) {
+    return guarded([&] {
+        FTL_REQUIRE(out, "null argument");
+        *out = ftl::jacobi_capped_read() + ftl::jacobi_capped_als_tu();
+    });
 }
 
 static int create_common(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, const double* const* U,

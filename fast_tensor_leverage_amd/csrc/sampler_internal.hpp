@@ -38,5 +38,8 @@ void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* 
 // update_sampler(j): rebuild tree j from the (aliased) factor j
 void sampler_rebuild(Sampler* s, uint32_t j, hipStream_t st);
 void sampler_set_seed(Sampler* s, uint64_t seed, uint64_t counter);
+// Eigensolves of als.hip's kernels that stopped at the Jacobi sweep cap on
+// the current device (waits for the device).
+uint64_t jacobi_capped_als_tu();
 
 }  // namespace ftl

@@ -57,6 +57,14 @@ const char* ftl_version(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int ftl_device_count(void);
 
+/* Eigensolves (computeM, the pseudo-inverses of the ALS and Larsen-Kolda
+ * Grams) on the current device that stopped at the 40-sweep cap of the
+ * cyclic Jacobi iteration before converging, since the library was loaded.
+ * LAPACK's dsyev, which the reference calls (efficient_krp_sampler.hpp:118,
+ * 168; common.h:257, 288), reports the same condition as info > 0.  Waits
+ * for the device.  No counterpart entry point in the reference. */
+int ftl_eigensolver_capped(uint64_t* out);
+
 /*
  * Sampler(matrices, J, R, "efficient") -- replaces PySampler's ctor
  * (cpp_ext/als_module.cpp:34-44) and EfficientKRPSampler's ctor
@@ -107,7 +115,11 @@ int ftl_sampler_draw_materialize(ftl_sampler* s, uint32_t j, uint64_t* samples_h
  * Device-resident KRPDrawSamples: every pointer is a device address, work is
  * enqueued on `stream` (hipStream_t; NULL = the handle's own stream) and the
  * call returns without synchronising.  h_dev / weights_dev / draws_dev may be
- * NULL.  samples_dev is N x J.
+ * NULL.  samples_dev is N x J.  A draw on another stream is ordered after the
+ * factor copies and tree builds queued before it on the handle's stream, and
+ * the next update_sampler (or fp32_levels change) waits for it on the device.
+ * Draws of one handle share its scratch buffers: draws on different streams
+ * must not run concurrently (order them with events or one stream).
  */
 int ftl_sampler_draw_device(ftl_sampler* s, uint32_t j, uint64_t* samples_dev, double* h_dev,
                             double* weights_dev, const double* draws_dev, void* stream);
@@ -207,7 +219,9 @@ int ftl_sparse_normsq(ftl_sparse* X, double* out);
  * (Jp x N uint64, cast to uint32) every nonzero equal to it on all modes but
  * j adds value * lhs[s] (lhs Jp x R) to result row (its mode-j coordinate).
  * result (result_rows x R, result_rows >= size of mode j) is zero-filled
- * first.  *found = number of matched nonzeros (found_count).
+ * first.  *found = number of matched nonzeros (found_count).  Samples may
+ * repeat (each repeat matches its nonzeros again); a total of 2^32 or more
+ * matches fails with FTL_EUNSUPPORTED before anything is written.
  */
 int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* samples_host,
                       const double* lhs_host, uint64_t R, double* result_host, uint64_t result_rows,

@@ -11,7 +11,9 @@ that hold at any size.
   Gram sum_i w_i h_i h_i^T a subspace embedding of G (spectral deviation of
   G^-1/2 S G^-1/2 from I below 0.15; about 2 sqrt(R/J) = 0.05 is expected).
 * Efficient sampler, FP32: index mismatch rate against the FP64 sampler on
-  identical uniforms at most 1e-3; h and weights by the same closed forms.
+  identical uniforms at most 1e-4, and every mismatch where the FP64
+  oracle's margin is below 1e-8 (the oracle also draws every sample the two
+  precisions disagree on); h and weights by the same closed forms.
 * Larsen-Kolda: leverage scores within 1e-10 relative of NumPy's
   u_i^T (U^T U)^-1 u_i; each sampled row is the inverse CDF of its uniform over
   the scores (larsen_kolda_sampler.hpp:102-136; checked against NumPy's cumsum
@@ -77,19 +79,25 @@ def test_efficient_sampler_full_size(ftl, oracle, factors):
         s.close()
         _check_efficient_closed_forms(U, grams, samples, h, w)
         out[prec] = samples
-    rate = np.any(out["fp32"] != out["fp64"], axis=0).mean()
+    mism = np.any(out["fp32"] != out["fp64"], axis=0)
+    rate = mism.mean()
     print(f"[gpu] full size: fp32 vs fp64 index mismatch rate {rate:.2e}", flush=True)
-    assert rate <= 1e-3
+    assert rate <= 1e-4
 
-    print(f"[gpu] full size: oracle on the first {J_ORACLE} samples", flush=True)
-    ref = oracle.OracleSampler(U, J_ORACLE, R)
-    sub = np.concatenate([draws[t * J: t * J + J_ORACLE] for t in range(2 * N)])
+    pick = np.union1d(np.arange(J_ORACLE), np.nonzero(mism)[0])
+    print(f"[gpu] full size: oracle on the first {J_ORACLE} samples and the {int(mism.sum())} mismatches",
+          flush=True)
+    ref = oracle.OracleSampler(U, pick.size, R)
+    sub = np.concatenate([draws[t * J + pick] for t in range(2 * N)])
     r = ref.draw(N + 1, sub)
     del ref
     sure = r["margins"] > 1e-10
-    gpu = out["fp64"][:, :J_ORACLE]
+    gpu = out["fp64"][:, pick]
     np.testing.assert_array_equal(gpu[:, sure], r["samples"][:, sure])
-    assert sure.mean() > 0.99
+    assert sure[:J_ORACLE].mean() > 0.99
+    m = r["margins"][np.isin(pick, np.nonzero(mism)[0])]
+    print(f"[gpu] full size: oracle margins of the fp32 mismatches {np.array2string(m, precision=2)}", flush=True)
+    assert np.all(m < 1e-8)
 
 
 def test_larsen_kolda_full_size(ftl, factors):

@@ -382,8 +382,12 @@ def test_lk_device_draws_and_updates_are_ordered(ftl, oracle):
     dev = torch.device("cuda:0")
     draws = rng.random(2 * 3 * J)
     td = torch.from_numpy(draws).to(dev)
-    st = torch.cuda.current_stream()
     outs = [torch.zeros((3, J), dtype=torch.int64, device=dev) for _ in range(4)]
+    # a stream of the test's own (a non-null handle), so the draws really run
+    # on another stream than the sampler's updates
+    st = torch.cuda.Stream()
+    assert st.cuda_stream != 0
+    st.wait_stream(torch.cuda.current_stream())
     for o in outs[:3]:
         s.draw_device(4, o, None, None, td, st)
     s.update_sampler(1, U1)  # must not overwrite mode 1's scores under the running draws

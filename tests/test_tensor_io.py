@@ -48,3 +48,15 @@ def test_tns_zero_based_and_errors(tmp_path):
     assert d2 == [3, 2]
     with pytest.raises(ValueError):
         read_tns(path, one_based=True)  # coordinate 0 in a 1-based file
+
+
+def test_als_prod_stop_rule():
+    """The stop rule of als.py:99-101: more than four fits, and the best of
+    the newest three below the best before them plus the tolerance."""
+    from fast_tensor_leverage_amd.als_driver import stalled
+    assert not stalled([0.1, 0.2, 0.3, 0.4], 0.0)  # four fits: never
+    assert not stalled([0.1, 0.2, 0.3, 0.4, 0.5], 0.0)  # improving
+    assert stalled([0.1, 0.5, 0.4, 0.45, 0.49], 0.0)  # no new best
+    assert stalled([0.1, 0.2, 0.40, 0.401, 0.402, 0.403], 0.01)  # gain below the tolerance
+    assert not stalled([0.1, 0.2, 0.40, 0.401, 0.402, 0.411], 0.01)
+    assert stalled([0.5, 0.4, 0.3, 0.2, 0.1, 0.6], 0.2)  # only the newest three count
