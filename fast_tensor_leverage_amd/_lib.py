@@ -24,6 +24,7 @@ SIGNATURES = {
     "ftl_version": ([], ctypes.c_char_p),
     "ftl_device_count": ([], _i32),
     "ftl_eigensolver_capped": ([_vp], _i32),
+    "ftl_tuning_lookup": ([ctypes.c_char_p, _i32, _u64, _u64, _u64, ctypes.c_char_p, _u64], _i32),
     "ftl_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_create_device": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_destroy": ([_vp], _i32),
@@ -147,6 +148,21 @@ def eigensolver_capped() -> int:
     out = ctypes.c_uint64()
     check(load().ftl_eigensolver_capped(ctypes.byref(out)), "eigensolver_capped")
     return int(out.value)
+
+
+def tuning_lookup(precision: int, R: int, depth: int, J: int, table_text: str = None):
+    """(table file, row index or -1, {option: value}) of the tuning-table row
+    for a shape; ``table_text`` parses that text instead of the loaded
+    table.  Host-only."""
+    buf = ctypes.create_string_buffer(4096)
+    txt = None if table_text is None else table_text.encode()
+    check(load().ftl_tuning_lookup(txt, int(precision), int(R), int(depth), int(J), buf, len(buf)), "tuning_lookup")
+    path, row, opts = buf.value.decode().split("\t")
+    kv = {}
+    for item in filter(None, opts.split(",")):
+        k, v = item.split("=")
+        kv[k] = int(v)
+    return path, int(row), kv
 
 
 def device_count() -> int:

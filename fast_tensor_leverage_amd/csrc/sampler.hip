@@ -8,9 +8,17 @@
 #include <type_traits>
 #include <vector>
 
+#include <dlfcn.h>
+
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
 #include "ftl_common.hpp"
 #include "sampler_internal.hpp"
 #include "sampler_kernels.hpp"
+#include "tuning.hpp"
 
 namespace ftl {
 
@@ -20,10 +28,102 @@ thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 const char* get_error() { return g_err.c_str(); }
 
+TuningTable parse_tuning(const std::string& text) {
+    TuningTable t;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream ls(line);
+        std::string prec, opts;
+        TuningRow r;
+        if (!(ls >> prec)) continue;
+        if (prec == "fp64") r.precision = FTL_FP64;
+        else if (prec == "fp32") r.precision = FTL_FP32;
+        else continue;
+        if (!(ls >> r.r_lo >> r.r_hi >> r.d_lo >> r.d_hi >> r.j_lo >> r.j_hi >> opts)) continue;
+        bool ok = true;
+        std::istringstream os(opts);
+        std::string kv;
+        while (std::getline(os, kv, ',')) {
+            const size_t eq = kv.find('=');
+            if (eq == std::string::npos || eq == 0) {
+                ok = false;
+                break;
+            }
+            char* end = nullptr;
+            const long long v = std::strtoll(kv.c_str() + eq + 1, &end, 10);
+            if (!end || *end != '\0') {
+                ok = false;
+                break;
+            }
+            r.opts.push_back({kv.substr(0, eq), (int64_t)v});
+        }
+        if (ok) t.rows.push_back(std::move(r));
+    }
+    return t;
+}
+
 namespace {
 
-// Levels below this many from the bottom are stored FP32 in FP32 mode.
-constexpr uint32_t kDefaultFp32Levels = 6;
+std::string library_dir() {
+    Dl_info info{};
+    if (dladdr((const void*)&library_dir, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        const size_t slash = p.rfind('/');
+        if (slash != std::string::npos) return p.substr(0, slash);
+    }
+    return ".";
+}
+
+}  // namespace
+
+const TuningTable& tuning_table() {
+    static TuningTable table;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char* env = std::getenv("FTL_TUNING_TABLE");
+        std::string path = env ? std::string(env) : library_dir() + "/ftl_tuning.tsv";
+        if (path.empty()) return;
+        std::ifstream f(path);
+        if (!f) return;
+        std::stringstream buf;
+        buf << f.rdbuf();
+        table = parse_tuning(buf.str());
+        table.path = path;
+    });
+    return table;
+}
+
+namespace {
+
+// Levels below this many from the bottom are stored FP32 in FP32 mode when
+// no tuning row sets "fp32_levels".
+constexpr uint32_t kDefaultFp32Levels = 5;
+
+// Depth (levels below the root) of the Gram tree over I rows with leaf size
+// min(R, I).
+inline uint32_t tree_depth(uint64_t I, uint64_t R) {
+    const Shape sh = make_shape(I, std::min<uint64_t>(R, I));
+    return (uint32_t)sh.lfill_level + (sh.node_count > sh.nodes_upto_lfill ? 1 : 0);
+}
+
+inline uint32_t max_depth(uint32_t N, const uint64_t* I, uint64_t R) {
+    uint32_t d = 0;
+    for (uint32_t k = 0; k < N; k++) d = std::max(d, tree_depth(I[k], R));
+    return d;
+}
+
+// fp32_levels of the tuning row for this shape (the default without one).
+inline uint32_t tuned_fp32_levels(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int precision) {
+    const TuningTable& t = tuning_table();
+    const int row = t.find(precision, R, max_depth(N, I, R), J);
+    if (row >= 0)
+        for (const auto& kv : t.rows[(size_t)row].opts)
+            if (kv.first == "fp32_levels" && kv.second > 0 && kv.second <= 64) return (uint32_t)kv.second;
+    return kDefaultFp32Levels;
+}
 
 // Zeroed entries after the last node of every tree beyond one packed node:
 // covers the over-read of the widest draw kernel (pairs of entries over 512
@@ -369,7 +469,28 @@ struct Sampler {
         }
     }
 
+    // Options of the tuning-table row for this shape (set before any buffer
+    // is allocated; later set_option calls override them).  Entries with an
+    // unknown key or a value set_option rejects are skipped.
+    int tuning_row = -1;
+    void apply_tuning(const uint64_t* I) {
+        const TuningTable& t = tuning_table();
+        tuning_row = t.find(precision, R, max_depth(N, I, R), J);
+        if (tuning_row < 0) return;
+        for (const auto& kv : t.rows[(size_t)tuning_row].opts) {
+            if (kv.first == "fp32_levels") {
+                if (kv.second > 0 && kv.second <= 64) fp32_levels = (uint32_t)kv.second;
+                continue;
+            }
+            try {
+                set_option(kv.first, kv.second);
+            } catch (const Error&) {
+            }
+        }
+    }
+
     int64_t get_option(const std::string& key) const {
+        if (key == "tuning_row") return tuning_row;
         if (key == "group_lanes") return sg_override;
         if (key == "grid_blocks") return grid_override;
         if (key == "mode_split") return mode_split;
@@ -437,6 +558,7 @@ struct Sampler {
         // entries from a node start (SG up to 512 lanes per sample), Tmax
         // being the packed size for the kernel's compile-time rank bound (R
         // rounded up to a power of two).
+        apply_tuning(I);
         uint64_t rmax = 8;
         while (rmax < R) rmax *= 2;
         const uint64_t pad = tri_size(rmax) + kTreePadExtra;
@@ -903,7 +1025,7 @@ Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R
 hipStream_t sampler_stream(Sampler* s) { return s->stream; }
 
 size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int precision, size_t align) {
-    return Sampler::footprint(N, I, J, R, precision, kDefaultFp32Levels, align);
+    return Sampler::footprint(N, I, J, R, precision, tuned_fp32_levels(N, I, J, R, precision), align);
 }
 
 Sampler* sampler_new_aliased(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, double* const* U_dev,
@@ -944,6 +1066,32 @@ int ftl_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int ftl_tuning_lookup(const char* table_text, int precision, uint64_t R, uint64_t depth, uint64_t J,
+                      char* out, uint64_t capacity) {
+    return guarded([&] {
+        FTL_REQUIRE(out && capacity > 0, "null output");
+        ftl::TuningTable parsed;
+        const ftl::TuningTable* t = &ftl::tuning_table();
+        if (table_text) {
+            parsed = ftl::parse_tuning(table_text);
+            t = &parsed;
+        }
+        const int row = t->find(precision, R, depth, J);
+        std::string s = t->path + "\t" + std::to_string(row) + "\t";
+        if (row >= 0) {
+            bool first = true;
+            for (const auto& kv : t->rows[(size_t)row].opts) {
+                if (!first) s += ",";
+                s += kv.first + "=" + std::to_string(kv.second);
+                first = false;
+            }
+        }
+        const size_t n = std::min<size_t>(s.size(), (size_t)capacity - 1);
+        std::memcpy(out, s.data(), n);
+        out[n] = '\0';
+    });
 }
 
 int ftl_eigensolver_capped(uint64_t* out) {

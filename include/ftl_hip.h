@@ -57,6 +57,16 @@ const char* ftl_version(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int ftl_device_count(void);
 
+/* The sampler's tuning table (no counterpart in the reference): launch
+ * shapes and the FP32/FP64 level split per (precision, R, tree depth, J)
+ * bucket, written by the package's autotuner into ftl_tuning.tsv beside
+ * libftl_hip.so (or the file named by FTL_TUNING_TABLE; empty disables it)
+ * and applied when a sampler is created.  Looks up the row for a shape in
+ * table_text (NULL: the loaded table) and writes "<file>\t<row>\t<k=v,...>"
+ * to out (row -1 when none applies).  Host-only: needs no GPU. */
+int ftl_tuning_lookup(const char* table_text, int precision, uint64_t R, uint64_t depth, uint64_t J,
+                      char* out, uint64_t capacity);
+
 /* Eigensolves (computeM, the pseudo-inverses of the ALS and Larsen-Kolda
  * Grams) on the current device that stopped at the 40-sweep cap of the
  * cyclic Jacobi iteration before converging, since the library was loaded.
@@ -165,7 +175,8 @@ void* ftl_sampler_stream(ftl_sampler* s);
  * default for a sampler with a stream of its own: computeM of a call runs on
  * a second stream into one of two buffer sets, overlapping the previous
  * call's draw kernels; 0: computeM and draw in order on one stream).
- * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree). */
+ * get_option also accepts "fp64_levels" (FP64 levels of mode 0's tree) and
+ * "tuning_row" (the tuning-table row applied at creation, -1 for none). */
 int ftl_sampler_set_option(ftl_sampler* s, const char* key, int64_t value);
 int ftl_sampler_get_option(ftl_sampler* s, const char* key, int64_t* value);
 

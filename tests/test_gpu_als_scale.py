@@ -138,13 +138,17 @@ def test_c5_full_size_properties(ftl):
         np.testing.assert_array_equal(ids, np.asarray(ref_ids, dtype=np.uint64))
         del skeys, perm, qk, lo, hi
 
-    # one sampled update of mode 0 against the dense sampled least squares
+    # one sampled update of mode 0 against the dense sampled least squares,
+    # after a sweep that moves the factors onto the tensor's support (from
+    # the random start almost no sampled tuple matches a nonzero)
     print("[gpu] C5: one execute_ds_als_update(0) against the sampled least squares", flush=True)
     lhs = ftl.PyLowRank(dims, R, seed=5)
     lhs.ten.renormalize_columns(-1)
-    U_init = [u.copy() for u in lhs.U]
     als = ftl.ALS(lhs.ten, X, host_sync=True)
-    als.initialize_ds_als(J, "efficient")
+    als.initialize_ds_als(J, "efficient", seed=17)
+    for j in range(N):
+        als.execute_ds_als_update(j, True, True)
+    U_init = [u.copy() for u in lhs.U]
     draws = np.random.default_rng(99).random(2 * N * J)
     smp = ftl.Sampler(U_init, J, R, "efficient")
     samples = np.zeros((N, J), dtype=np.uint64)
@@ -156,6 +160,7 @@ def test_c5_full_size_properties(ftl):
     Uj = lhs.U[0]
     touched = np.nonzero(np.any(Uj != 0, axis=1))[0]
     print(f"[gpu] C5: {touched.size} touched rows of mode 0, found {als.last_stats()['found']}", flush=True)
+    assert touched.size > 100_000
     rows = rng.choice(touched, size=min(256, touched.size), replace=False)
     fkeys = _pack(torch, idx, dims, [0, 1, 2])
     sfk, sperm = torch.sort(fkeys)
