@@ -40,6 +40,15 @@ N_MODES = 3
 MAX_FACTOR_ROWS = 2 ** 23
 
 
+def tree_depth(I, R):
+    """Levels below the root of the Gram tree over I rows with leaf size
+    min(R, I) (partition_tree.hpp:83-105): the tuning table's depth key."""
+    F = min(R, I)
+    leaves = -(-I // F)
+    lfill = leaves.bit_length() - 1
+    return lfill + (0 if leaves == 1 << lfill else 1)
+
+
 def representative_rows(R, depth):
     """Rows per factor whose tree (leaf size R) has `depth` levels below the
     root, capped so that one factor stays at most 2^23 rows."""
