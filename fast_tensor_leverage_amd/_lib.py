@@ -51,6 +51,7 @@ SIGNATURES = {
     "ftl_sparse_mttkrp": ([_vp, _u32, _u64, _vp, _vp, _u64, _vp, _u64, _vp], _i32),
     "ftl_sparse_lookup": ([_vp, _u32, _u64, _vp, _vp, _vp, _u64, _vp], _i32),
     "ftl_sparse_residual_normsq": ([_vp, _u64, _vp, _vp, _vp, _vp], _i32),
+    "ftl_sparse_residual_normsq_noalloc": ([_vp, _u64, _vp, _vp, _vp, _vp, _vp], _i32),
     "ftl_als_create": ([_vp, _u32, _vp, _u64, _vp, _vp], _i32),
     "ftl_als_destroy": ([_vp], _i32),
     "ftl_als_initialize_ds": ([_vp, _u64, _i32], _i32),

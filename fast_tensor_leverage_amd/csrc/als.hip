@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1314,6 +1315,114 @@ using ftl::guarded;
 
 namespace ftl {
 uint64_t jacobi_capped_als_tu() { return jacobi_capped_read(); }
+
+// ---------------------------------------------------------------------------
+// Allocation-free residual: compute_residual_normsq on device factors with
+// every intermediate in `scratch`, a file-scope device array of
+// kNoAllocScratchDoubles doubles (9 MiB, held by each GPU the library runs
+// on until the process ends).  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (scratch_base); calls
+// that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kNoAllocScratchDoubles = 9ull << 17;  // 1,179,648 doubles = 9 MiB
+
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_base;  // per device, null until first looked up
+
+double* scratch_base() {
+    int dev = 0;
+    FTL_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_base.size()) g_scratch_base.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_base[(size_t)dev]) {
+        void* p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !p)
+            throw Error{FTL_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc)};
+        g_scratch_base[(size_t)dev] = (double*)p;
+    }
+    return g_scratch_base[(size_t)dev];
+}
+
+__global__ void k_add_pair(const double* __restrict__ two, double* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = two[0] + two[1];
+}
+
+template <int RMAX>
+void noalloc_gram(const double* U, uint64_t n, uint32_t R, double* parts, unsigned nb, hipStream_t st) {
+    hipLaunchKernelGGL((k_gram_partial<256, RMAX>), dim3(nb), dim3(256), 0, st, U, (const uint32_t*)nullptr, n, R,
+                       parts);
+    FTL_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+// The same kernels, in the same order and with the same grids, as
+// Als::residual_normsq (so the result equals ftl_sparse_residual_normsq's
+// bitwise), with the Gram partials of one mode at a time.
+void residual_normsq_noalloc(SparseDev& X, uint64_t R, const uint64_t* I, const double* sigma,
+                             const double* const* U, double* out, hipStream_t st) {
+    FTL_REQUIRE(R >= 1 && R <= 128, "R must be in [1, 128]");
+    FTL_REQUIRE(I && U && sigma && out, "null argument");
+    const uint32_t N = X.N;
+    const uint64_t T = R * (R + 1) / 2;
+    uint64_t imax = 1;
+    for (uint32_t k = 0; k < N; k++) {
+        FTL_REQUIRE(U[k], "null factor");
+        FTL_REQUIRE(I[k] >= X.dims[k], "factor has fewer rows than the tensor's mode size");
+        imax = std::max(imax, I[k]);
+    }
+    const unsigned nb = (unsigned)std::min<uint64_t>(kGramBlocksMax, std::max<uint64_t>(1, (imax + 4095) / 4096));
+    // layout: [scal 4][resid_part kResidBlocks][gram_red N T][gram_parts nb T]
+    const uint64_t need = 4 + kResidBlocks + (uint64_t)N * T + (uint64_t)nb * T;
+    if (need > kNoAllocScratchDoubles)
+        throw Error{FTL_EUNSUPPORTED, "the residual needs " + std::to_string(need * 8) + " bytes of scratch for R = " +
+                                          std::to_string(R) + " and " + std::to_string(imax) +
+                                          " factor rows; the allocation-free entry point holds " +
+                                          std::to_string(kNoAllocScratchDoubles * 8)};
+    double* base = scratch_base();
+    double* scal = base;
+    double* resid_part = base + 4;
+    double* gram_red = resid_part + kResidBlocks;
+    double* gram_parts = gram_red + (uint64_t)N * T;
+    FactorPtrs F{};
+    for (uint32_t k = 0; k < N; k++) F.U[k] = U[k];
+    uint32_t s = 0;
+    for (uint32_t k = 1; k < N; k++)
+        if (X.dims[k] < X.dims[s]) s = k;
+    const uint32_t Ru = (uint32_t)R;
+    if (R <= 32)
+        hipLaunchKernelGGL((k_residual_partial<32, 1, 256>), dim3(kResidBlocks), dim3(256), 0, st, X.keys[s], X.row[s],
+                           X.val[s], X.nnz, X.layout[s], F, sigma, Ru, resid_part);
+    else if (R <= 64)
+        hipLaunchKernelGGL((k_residual_partial<64, 1, 256>), dim3(kResidBlocks), dim3(256), 0, st, X.keys[s], X.row[s],
+                           X.val[s], X.nnz, X.layout[s], F, sigma, Ru, resid_part);
+    else
+        hipLaunchKernelGGL((k_residual_partial<64, 2, 256>), dim3(kResidBlocks), dim3(256), 0, st, X.keys[s], X.row[s],
+                           X.val[s], X.nnz, X.layout[s], F, sigma, Ru, resid_part);
+    FTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, resid_part, kResidBlocks, scal);
+    FTL_LAUNCH_CHECK();
+    for (uint32_t k = 0; k < N; k++) {
+        if (R <= 8) noalloc_gram<8>(U[k], I[k], Ru, gram_parts, nb, st);
+        else if (R <= 16) noalloc_gram<16>(U[k], I[k], Ru, gram_parts, nb, st);
+        else if (R <= 32) noalloc_gram<32>(U[k], I[k], Ru, gram_parts, nb, st);
+        else if (R <= 64) noalloc_gram<64>(U[k], I[k], Ru, gram_parts, nb, st);
+        else noalloc_gram<128>(U[k], I[k], Ru, gram_parts, nb, st);
+        hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(T, 256)), dim3(256), 0, st, gram_parts, nb, (uint32_t)T, 1u,
+                           gram_red + (uint64_t)k * T);
+        FTL_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_chain_final, dim3(1), dim3(256), 0, st, gram_red, N, Ru, sigma, scal + 1);
+    FTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_add_pair, dim3(1), dim3(64), 0, st, scal, out);
+    FTL_LAUNCH_CHECK();
+}
+
 }  // namespace ftl
 
 extern "C" {
@@ -1449,6 +1558,15 @@ static int als_create_common(ftl_sparse* X, uint32_t N, const uint64_t* I, uint6
         std::unique_ptr<ftl_als> a(new ftl_als());
         a->impl.init(X ? &X->impl : nullptr, N, I, R, U_host);
         *out = a.release();
+    });
+}
+
+int ftl_sparse_residual_normsq_noalloc(ftl_sparse* X, uint64_t R, const uint64_t* I, const double* sigma_dev,
+                                       const double* const* U_dev, double* out_dev, void* stream) {
+    return guarded([&] {
+        FTL_REQUIRE(X, "null sparse tensor");
+        hipStream_t st = stream ? (hipStream_t)stream : X->impl.stream;
+        ftl::residual_normsq_noalloc(X->impl, R, I, sigma_dev, U_dev, out_dev, st);
     });
 }
 

@@ -258,6 +258,23 @@ int ftl_sparse_residual_normsq(ftl_sparse* X, uint64_t R, const uint64_t* I, con
                                const double* const* U_host, double* out);
 
 /*
+ * Allocation-free ftl_sparse_residual_normsq on device arrays.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  sigma_dev (R), U_dev[k] (I[k] x R FP64, I[k] >= size of mode
+ * k; the pointer list itself is a host array) and out_dev (one double) are
+ * device addresses; the work is enqueued on `stream` (NULL: the tensor's
+ * stream) without synchronising.  Intermediates live in a 9 MiB device array
+ * of the library (one per GPU, held until the process ends): a shape whose
+ * intermediates exceed it (4 + 4096 + (N + nb) R(R+1)/2 doubles, nb =
+ * min(512, ceil(max I[k] / 4096))) fails with FTL_EUNSUPPORTED before any
+ * work is queued.  Calls that overlap on one GPU share that array and race.
+ * The result equals ftl_sparse_residual_normsq's bitwise.
+ */
+int ftl_sparse_residual_normsq_noalloc(ftl_sparse* X, uint64_t R, const uint64_t* I, const double* sigma_dev,
+                                       const double* const* U_dev, double* out_dev, void* stream);
+
+/*
  * ALS(LowRankTensor(R, U), X) -- replaces the LowRankTensor ctor
  * (low_rank_tensor.hpp:52-68, sigma = 1) and the ALS ctor (als.hpp:31-37),
  * bound at als_module.cpp:85-98, 111-115.  Copies the factors U_host[k]
