@@ -46,6 +46,7 @@ SIGNATURES = {
     "ftl_sparse_create": ([_u32, _u64, _vp, _vp, _vp, _vp], _i32),
     "ftl_sparse_create_device": ([_u32, _u64, _vp, _vp, _vp, _vp], _i32),
     "ftl_sparse_destroy": ([_vp], _i32),
+    "ftl_sparse_set_lookup": ([_vp, _i32], _i32),
     "ftl_sparse_info": ([_vp, _vp], _i32),
     "ftl_sparse_normsq": ([_vp, _vp], _i32),
     "ftl_sparse_mttkrp": ([_vp, _u32, _u64, _vp, _vp, _u64, _vp, _u64, _vp], _i32),
@@ -92,6 +93,12 @@ SIGNATURES = {
     "ftl_server_factor": ([_vp, _u64, _u32, _vp], _i32),
     "ftl_server_cancel": ([_vp, _u64], _i32),
     "ftl_server_stats": ([_vp, _vp], _i32),
+    "ftl_server_add_factors": ([_vp, _u32, _vp, _u64, _vp, _i32, _i32, _u64, _vp], _i32),
+    "ftl_server_add_sampler": ([_vp, _vp, _vp], _i32),
+    "ftl_server_update_factor": ([_vp, _u64, _u32, _vp, _i32], _i32),
+    "ftl_server_remove_factors": ([_vp, _u64], _i32),
+    "ftl_server_submit_on": ([_vp, _u64, _u64, _u32, _u64, _vp], _i32),
+    "ftl_server_wait": ([_vp, _u64, ctypes.c_double, _vp], _i32),
 }
 
 _lib = None
@@ -171,5 +178,14 @@ def device_count() -> int:
 
 
 def require_device():
-    if device_count() < 1:
-        raise FTLError("no HIP device visible: the MI355X path needs a GPU")
+    n = device_count()
+    if n < 1:
+        # a second opinion and the visibility settings, for the report
+        try:
+            import torch
+            tn = torch.cuda.device_count()
+        except Exception as e:  # noqa: BLE001
+            tn = f"unavailable ({e})"
+        env = {k: os.environ.get(k) for k in ("HIP_VISIBLE_DEVICES", "ROCR_VISIBLE_DEVICES", "CUDA_VISIBLE_DEVICES")}
+        raise FTLError(f"no HIP device visible: the MI355X path needs a GPU (hipGetDeviceCount gave {n}; "
+                       f"torch.cuda.device_count() = {tn}; {env})")

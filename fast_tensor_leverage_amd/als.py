@@ -33,14 +33,16 @@ class SparseTensor:
     """``SparseTensor(idx, values, "sort")`` (sparse_tensor.hpp:46-77).
 
     idx: nnz x N uint32 (row-major), values: nnz float64.  The per-mode sorted
-    lookups are built in HBM.  ``method`` "sort" (the reference's only
-    accepted value, sparse_tensor.hpp:57-67) or "hash" (same device lookup).
+    lookups are built in HBM.  ``method`` "sort" (binary search of the sorted
+    keys, SortIdxLookup) or "hash" (a device open-addressing coordinate hash
+    per mode over the same sorted nonzeros, the device form of the
+    reference's HashIdxLookup, hash_lookup.hpp:13-162; the same matches).
     ``dims`` optionally gives the mode sizes (default max coordinate + 1).
     """
 
     def __init__(self, idx, values, method: str = "sort", dims=None):
         if method not in ("sort", "hash"):
-            raise ValueError(f"unknown lookup method '{method}' (expected 'sort')")
+            raise ValueError(f"unknown lookup method '{method}' (expected 'sort' or 'hash')")
         _require_array(idx, np.uint32, 2, "idx")
         _require_array(values, np.float64, 1, "values")
         nnz, N = idx.shape
@@ -67,6 +69,9 @@ class SparseTensor:
         self.nnz = nnz
         self.dims = [int(d) for d in info[2:]]
         self.last_found = 0
+        self.method = method
+        if method == "hash":
+            _lib.check(L.ftl_sparse_set_lookup(self._h, 1), "SparseTensor (hash lookup)")
 
     @classmethod
     def from_device(cls, idx, values, dims=None, method: str = "sort"):
@@ -97,6 +102,8 @@ class SparseTensor:
         _lib.check(_lib.load().ftl_sparse_info(obj._h, _ptr(info)))
         obj.N, obj.nnz, obj.dims, obj.last_found = N, nnz, [int(d) for d in info[2:]], 0
         obj.method = method
+        if method == "hash":
+            _lib.check(_lib.load().ftl_sparse_set_lookup(obj._h, 1), "SparseTensor (hash lookup)")
         return obj
 
     def __del__(self):

@@ -248,6 +248,41 @@ struct SparseDev {
         uint64_t D = 0, W = 0;
     };
     RowOrder rows_of[kMaxModes];
+    // lookup method of the probes: 0 = binary search of the sorted keys,
+    // 1 = the per-mode coordinate hash (built by use_hash)
+    int method = 0;
+    HashSlot* table[kMaxModes] = {};
+    uint64_t table_mask[kMaxModes] = {};
+
+    // Builds the coordinate hash of every mode (once) and probes with it.
+    void use_hash() {
+        for (uint32_t j = 0; j < N; j++)
+            if (layout[j].total_bits >= 64)
+                throw Error{FTL_EUNSUPPORTED, "the coordinate hash needs keys of fewer than 64 bits (mode " +
+                                                  std::to_string(j) + " uses 64)"};
+        unsigned long long* cnt_dev = mem.alloc<unsigned long long>(1);
+        for (uint32_t j = 0; j < N; j++) {
+            if (table[j]) continue;
+            FTL_HIP_CHECK(hipMemsetAsync(cnt_dev, 0, sizeof(unsigned long long), stream));
+            hipLaunchKernelGGL(k_count_heads, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, stream, keys[j], nnz,
+                               cnt_dev);
+            FTL_LAUNCH_CHECK();
+            unsigned long long distinct = 0;
+            FTL_HIP_CHECK(hipMemcpyAsync(&distinct, cnt_dev, sizeof(distinct), hipMemcpyDeviceToHost, stream));
+            FTL_HIP_CHECK(hipStreamSynchronize(stream));
+            uint64_t slots = 2;
+            while (slots < 2 * (uint64_t)distinct) slots *= 2;
+            table[j] = mem.alloc<HashSlot>(slots);
+            table_mask[j] = slots - 1;
+            FTL_HIP_CHECK(hipMemsetAsync(table[j], 0xff, slots * sizeof(HashSlot), stream));
+            hipLaunchKernelGGL(k_hash_build, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, stream, keys[j], nnz,
+                               table[j], table_mask[j]);
+            FTL_LAUNCH_CHECK();
+        }
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        mem.release(cnt_dev);
+        method = 1;
+    }
 
     ~SparseDev() {
         if (stream) {
@@ -417,6 +452,8 @@ struct SparseDev {
         for (uint32_t k = 0; k < N; k++) a.dims[k] = dims[k];
         a.lo = m.lo;
         a.cnt = m.cnt;
+        a.table = method == 1 ? table[j] : nullptr;
+        a.mask = table_mask[j];
         return a;
     }
 };
@@ -1458,6 +1495,15 @@ int ftl_sparse_info(ftl_sparse* X, uint64_t* out) {
         out[0] = X->impl.N;
         out[1] = X->impl.nnz;
         for (uint32_t k = 0; k < X->impl.N; k++) out[2 + k] = X->impl.dims[k];
+    });
+}
+
+int ftl_sparse_set_lookup(ftl_sparse* X, int method) {
+    return guarded([&] {
+        FTL_REQUIRE(X, "null sparse tensor");
+        FTL_REQUIRE(method == 0 || method == 1, "lookup method must be 0 (sort) or 1 (hash)");
+        if (method == 1) X->impl.use_hash();
+        else X->impl.method = 0;
     });
 }
 

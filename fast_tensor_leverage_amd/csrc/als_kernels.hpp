@@ -439,6 +439,63 @@ __global__ void __launch_bounds__(BS) k_pinv_partials(const double* __restrict__
 // (sort_lookup.hpp:94-103).  A coordinate outside the tensor's range cannot
 // match (its field would not hold it) and gives an empty range.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Coordinate hash of mode j's lookup: an open-addressing table (linear
+// probing, load factor <= 1/2, a power-of-two number of slots) holding, for
+// every distinct key of the sorted nonzeros, the key and its run [lo,
+// lo + cnt) in the sorted arrays -- the range a binary search would find.
+// Empty slots hold key kHashEmpty, which no key of fewer than 64 bits takes.
+// ---------------------------------------------------------------------------
+struct HashSlot {
+    uint64_t key;
+    uint32_t lo, cnt;
+};
+constexpr uint64_t kHashEmpty = ~0ull;
+
+// splitmix64's finalizer (Steele, Lea and Flood, OOPSLA 2014)
+__device__ inline uint64_t hash_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Number of distinct keys (run heads) of the sorted keys, added to *count.
+__global__ void k_count_heads(const uint64_t* __restrict__ keys, uint64_t nnz, unsigned long long* count) {
+    uint64_t c = 0;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (uint64_t)gridDim.x * blockDim.x)
+        c += (p == 0 || keys[p - 1] != keys[p]) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// Inserts every run head p of the sorted keys with its run length (the
+// first later position holding a larger key, by binary search).
+__global__ void k_hash_build(const uint64_t* __restrict__ keys, uint64_t nnz, HashSlot* __restrict__ table,
+                             uint64_t mask) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = keys[p];
+        if (p > 0 && keys[p - 1] == key) continue;
+        uint64_t l = p + 1, h = nnz;
+        while (l < h) {  // first position with keys[q] > key
+            const uint64_t m = (l + h) >> 1;
+            if (keys[m] <= key) l = m + 1;
+            else h = m;
+        }
+        uint64_t slot = hash_mix(key) & mask;
+        for (;;) {
+            const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot].key,
+                                                      (unsigned long long)kHashEmpty, (unsigned long long)key);
+            if (prev == kHashEmpty) {
+                table[slot].lo = (uint32_t)p;
+                table[slot].cnt = (uint32_t)(l - p);
+                break;
+            }
+            slot = (slot + 1) & mask;
+        }
+    }
+}
+
 struct ProbeArgs {
     const uint32_t* tup;       // n x N, row-major
     const uint32_t* n_dev;     // J' on the device, or null (n_host)
@@ -450,6 +507,8 @@ struct ProbeArgs {
     uint64_t dims[kMaxModes];  // coordinate bound per mode
     uint32_t* lo;
     uint32_t* cnt;
+    const HashSlot* table;     // mode j's coordinate hash, or null (binary search)
+    uint64_t mask;             // table slots - 1
 };
 
 __global__ void k_probe(ProbeArgs a) {
@@ -466,7 +525,19 @@ __global__ void k_probe(ProbeArgs a) {
                 ok &= t[k] < a.dims[k];
                 key |= (uint64_t)t[k] << a.L.shift[k];
             }
-            if (ok) {
+            if (ok && a.table) {
+                uint64_t slot = hash_mix(key) & a.mask;
+                for (;;) {
+                    const HashSlot e = a.table[slot];
+                    if (e.key == key) {
+                        lo = e.lo;
+                        c = e.cnt;
+                        break;
+                    }
+                    if (e.key == kHashEmpty) break;
+                    slot = (slot + 1) & a.mask;
+                }
+            } else if (ok) {
                 uint64_t l = 0, h = a.nnz;
                 while (l < h) {  // first position with keys[p] >= key
                     const uint64_t m = (l + h) >> 1;

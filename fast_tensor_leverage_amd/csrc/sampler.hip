@@ -549,7 +549,10 @@ struct Sampler {
             // eigenvectors when R <= 64: at most 132 KB of dynamic LDS
             const void* cm[] = {(const void*)k_gram_pinv<256>, (const void*)k_eigen_modes<256>,
                                 (const void*)k_gram_pinv<512>, (const void*)k_eigen_modes<512>,
-                                (const void*)k_gram_pinv<1024>, (const void*)k_eigen_modes<1024>};
+                                (const void*)k_gram_pinv<1024>, (const void*)k_eigen_modes<1024>,
+                                (const void*)k_gram_pinv_batched<256>, (const void*)k_eigen_modes_batched<256>,
+                                (const void*)k_gram_pinv_batched<512>, (const void*)k_eigen_modes_batched<512>,
+                                (const void*)k_gram_pinv_batched<1024>, (const void*)k_eigen_modes_batched<1024>};
             for (const void* f : cm)
                 FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
             attrs_set = true;
@@ -856,37 +859,11 @@ struct Sampler {
         });
     }
 
-    void draw_device(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws, hipStream_t st) {
-        FTL_REQUIRE(samples != nullptr, "null samples buffer");
-        if (!st) st = stream;
-        std::vector<int> order = sampled_modes(j);
-        FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
-        const bool foreign = st != stream;
-        if (foreign) {
-            if (!handle_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&handle_ev, hipEventDisableTiming));
-            if (!foreign_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&foreign_ev, hipEventDisableTiming));
-            FTL_HIP_CHECK(hipEventRecord(handle_ev, stream));
-            FTL_HIP_CHECK(hipStreamWaitEvent(st, handle_ev, 0));
-        }
-        std::array<hipEvent_t, 5> evs{};
-        if (timing)
-            for (auto& e : evs) e = take_event();
-        const bool overlap = overlap_cm && n_msets > 1;
-        const MSet& ms = msets[overlap ? (mset_next++ & 1) : 0];
-        if (overlap) {
-            if (trees_dirty) {
-                FTL_HIP_CHECK(hipEventRecord(trees_ev, stream));
-                trees_dirty = false;
-            }
-            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, trees_ev, 0));
-            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, ms.draw_done, 0));
-            compute_m(j, cm_stream, ms, timing ? evs.data() : nullptr);
-            FTL_HIP_CHECK(hipEventRecord(ms.cm_done, cm_stream));
-            FTL_HIP_CHECK(hipStreamWaitEvent(st, ms.cm_done, 0));
-        } else {
-            compute_m(j, st, ms, timing ? evs.data() : nullptr);
-        }
-        if (timing) FTL_HIP_CHECK(hipEventRecord(evs[3], st));
+    // Kernel arguments of one draw over the sampled modes `order` reading the
+    // computeM outputs of set ms (all modes in one launch; the caller sets
+    // mi_begin / mi_end / h_state for per-mode launches).
+    DrawArgs make_draw_args(const std::vector<int>& order, uint64_t* samples, double* h, double* w,
+                            const double* draws, const MSet& ms) const {
         DrawArgs a{};
         for (uint32_t k = 0; k < N; k++) {
             const ModeHost& m = modes[k];
@@ -924,6 +901,44 @@ struct Sampler {
         a.bucket = bucket;
         a.blist = blist;
         a.boff = boff;
+        a.mi_begin = 0;
+        a.mi_end = a.m;
+        a.h_state = nullptr;
+        return a;
+    }
+
+    void draw_device(uint32_t j, uint64_t* samples, double* h, double* w, const double* draws, hipStream_t st) {
+        FTL_REQUIRE(samples != nullptr, "null samples buffer");
+        if (!st) st = stream;
+        std::vector<int> order = sampled_modes(j);
+        FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
+        const bool foreign = st != stream;
+        if (foreign) {
+            if (!handle_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&handle_ev, hipEventDisableTiming));
+            if (!foreign_ev) FTL_HIP_CHECK(hipEventCreateWithFlags(&foreign_ev, hipEventDisableTiming));
+            FTL_HIP_CHECK(hipEventRecord(handle_ev, stream));
+            FTL_HIP_CHECK(hipStreamWaitEvent(st, handle_ev, 0));
+        }
+        std::array<hipEvent_t, 5> evs{};
+        if (timing)
+            for (auto& e : evs) e = take_event();
+        const bool overlap = overlap_cm && n_msets > 1;
+        const MSet& ms = msets[overlap ? (mset_next++ & 1) : 0];
+        if (overlap) {
+            if (trees_dirty) {
+                FTL_HIP_CHECK(hipEventRecord(trees_ev, stream));
+                trees_dirty = false;
+            }
+            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, trees_ev, 0));
+            FTL_HIP_CHECK(hipStreamWaitEvent(cm_stream, ms.draw_done, 0));
+            compute_m(j, cm_stream, ms, timing ? evs.data() : nullptr);
+            FTL_HIP_CHECK(hipEventRecord(ms.cm_done, cm_stream));
+            FTL_HIP_CHECK(hipStreamWaitEvent(st, ms.cm_done, 0));
+        } else {
+            compute_m(j, st, ms, timing ? evs.data() : nullptr);
+        }
+        if (timing) FTL_HIP_CHECK(hipEventRecord(evs[3], st));
+        DrawArgs a = make_draw_args(order, samples, h, w, draws, ms);
         if (mode_split && h_state) {
             a.h_state = h_state;
             for (int mi = 0; mi < a.m; mi++) {
@@ -1024,6 +1039,108 @@ Sampler* sampler_new_async(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R
 
 hipStream_t sampler_stream(Sampler* s) { return s->stream; }
 
+namespace {
+
+constexpr size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
+
+// Byte layout of one packed launch's arguments (host staging and device copy).
+struct PackLayout {
+    size_t draw = 0, cm = 0, off = 0, breq = 0, bpos = 0, total = 0;
+    PackLayout(uint32_t n, uint32_t blocks) {
+        draw = 0;
+        cm = align_up(draw + (size_t)n * sizeof(DrawArgs), 256);
+        off = align_up(cm + (size_t)n * sizeof(ComputeMArgs), 256);
+        breq = align_up(off + (size_t)(n + 1) * sizeof(uint64_t), 256);
+        bpos = align_up(breq + (size_t)blocks * sizeof(uint32_t), 256);
+        total = align_up(bpos + (size_t)blocks * sizeof(uint32_t), 256);
+    }
+};
+
+}  // namespace
+
+size_t packed_args_bytes(uint32_t max_req) { return PackLayout(max_req, max_req * kMaxModes).total; }
+
+uint64_t sampler_capacity(const Sampler* s) { return s->J; }
+
+void sampler_set_option(Sampler* s, const char* key, int64_t value) { s->set_option(key, value); }
+
+bool sampler_packable(const Sampler* s) {
+    return s->R <= 64 && (s->draw_kernel == 0 || s->draw_kernel == 3) && s->N >= 1;
+}
+
+uint64_t sampler_pack_key(const Sampler* s) {
+    return (s->R << 8) | ((uint64_t)s->precision << 4) | (uint64_t)(s->sg_override >> 3) |
+           ((uint64_t)s->leaf32 << 40) | ((uint64_t)(s->jacobi_override ? s->jacobi_override : jacobi_threads_for(s->R))
+                                          << 44);
+}
+
+void sampler_draw_packed(const PackItem* items, uint32_t n, void* dev_args, void* host_stage, hipStream_t st) {
+    FTL_REQUIRE(n >= 1, "empty pack");
+    const Sampler* s0 = items[0].s;
+    const uint64_t key = sampler_pack_key(s0);
+    std::vector<DrawArgs> da(n);
+    std::vector<ComputeMArgs> cm(n);
+    std::vector<uint32_t> breq, bpos;
+    std::vector<uint64_t> off(n + 1, 0);
+    uint32_t sg = 0;
+    s0->with_cfg([&](auto, auto sgc) { sg = (uint32_t)decltype(sgc)::value; });
+    const uint64_t gpw = 64 / sg;
+    for (uint32_t i = 0; i < n; i++) {
+        Sampler* s = items[i].s;
+        FTL_REQUIRE(sampler_packable(s) && sampler_pack_key(s) == key, "draws of one pack need the same R, precision "
+                                                                         "and launch options");
+        std::vector<int> order = s->sampled_modes(items[i].j);
+        FTL_REQUIRE(!order.empty(), "no mode left to sample (N == 1 and j == 0)");
+        cm[i] = s->computeM_args(order, s->msets[0]);
+        da[i] = s->make_draw_args(order, items[i].samples, nullptr, nullptr, nullptr, s->msets[0]);
+        FTL_REQUIRE(items[i].J >= 1 && items[i].J <= s->J, "a packed draw asks for more samples than its sampler holds");
+        da[i].J = items[i].J;
+        s->call++;
+        for (size_t t = 0; t < order.size(); t++) {
+            breq.push_back(i);
+            bpos.push_back((uint32_t)t);
+        }
+        off[i + 1] = off[i] + (items[i].J + gpw - 1) / gpw;
+    }
+    const uint32_t blocks = (uint32_t)breq.size();
+    const PackLayout L(n, blocks);
+    char* hs = (char*)host_stage;
+    std::memcpy(hs + L.draw, da.data(), n * sizeof(DrawArgs));
+    std::memcpy(hs + L.cm, cm.data(), n * sizeof(ComputeMArgs));
+    std::memcpy(hs + L.off, off.data(), (n + 1) * sizeof(uint64_t));
+    std::memcpy(hs + L.breq, breq.data(), blocks * sizeof(uint32_t));
+    std::memcpy(hs + L.bpos, bpos.data(), blocks * sizeof(uint32_t));
+    char* dv = (char*)dev_args;
+    FTL_HIP_CHECK(hipMemcpyAsync(dv, hs, L.total, hipMemcpyHostToDevice, st));
+    const ComputeMArgs* cm_dev = (const ComputeMArgs*)(dv + L.cm);
+    const uint64_t R = s0->R;
+    const uint64_t jac = 3 * (R + 1) * R <= (uint64_t)kJacobiLdsDoubles ? 3 * (R + 1) * R : (R + 1) * R;
+    const size_t lds = jac * sizeof(double);
+    const uint32_t bs = s0->jacobi_override ? s0->jacobi_override : jacobi_threads_for(R);
+    auto cm_launch = [&](auto bsc) {
+        constexpr int BS = decltype(bsc)::value;
+        hipLaunchKernelGGL(k_gram_pinv_batched<BS>, dim3(n), dim3(BS), lds, st, cm_dev);
+        FTL_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_eigen_modes_batched<BS>, dim3(blocks), dim3(BS), lds, st, cm_dev,
+                           (const uint32_t*)(dv + L.breq), (const uint32_t*)(dv + L.bpos));
+        FTL_HIP_CHECK(hipGetLastError());
+    };
+    if (bs == 256) cm_launch(std::integral_constant<int, 256>{});
+    else if (bs == 512) cm_launch(std::integral_constant<int, 512>{});
+    else cm_launch(std::integral_constant<int, 1024>{});
+    s0->with_cfg([&](auto rmax, auto sgc) {
+        constexpr int RM = decltype(rmax)::value, SG = decltype(sgc)::value, BS = 256;
+        const uint64_t blocks_d = std::max<uint64_t>(1, std::min<uint64_t>((off[n] + BS / 64 - 1) / (BS / 64), 1u << 20));
+        if (s0->precision == FTL_FP32)
+            hipLaunchKernelGGL((k_krp_draw_packed<RM, SG, BS, true>), dim3((unsigned)blocks_d), dim3(BS), 0, st,
+                               (const DrawArgs*)(dv + L.draw), (const uint64_t*)(dv + L.off), n, (int)R);
+        else
+            hipLaunchKernelGGL((k_krp_draw_packed<RM, SG, BS, false>), dim3((unsigned)blocks_d), dim3(BS), 0, st,
+                               (const DrawArgs*)(dv + L.draw), (const uint64_t*)(dv + L.off), n, (int)R);
+        FTL_HIP_CHECK(hipGetLastError());
+    });
+}
+
 size_t sampler_footprint(uint32_t N, const uint64_t* I, uint64_t J, uint64_t R, int precision, size_t align) {
     return Sampler::footprint(N, I, J, R, precision, tuned_fp32_levels(N, I, J, R, precision), align);
 }
@@ -1045,6 +1162,20 @@ void sampler_draw(Sampler* s, uint32_t j, uint64_t* samples, double* h, double* 
 void sampler_rebuild(Sampler* s, uint32_t j, hipStream_t st) {
     FTL_REQUIRE(j < s->N, "update_sampler: mode index out of range");
     s->build_mode(j, st);
+}
+
+SamplerView sampler_view(ftl_sampler* h) {
+    const Sampler& s = h->impl;
+    SamplerView v{};
+    v.N = s.N;
+    for (uint32_t k = 0; k < s.N; k++) {
+        v.I[k] = s.modes[k].I;
+        v.U[k] = s.modes[k].U64;
+    }
+    v.R = s.R;
+    v.J = s.J;
+    v.precision = s.precision;
+    return v;
 }
 
 void sampler_set_seed(Sampler* s, uint64_t seed, uint64_t counter) {

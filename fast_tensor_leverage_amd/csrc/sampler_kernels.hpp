@@ -287,11 +287,9 @@ __device__ inline void fill_pair_table(uint8_t* pa, uint8_t* pb, int R) {
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
+__device__ __forceinline__ void gram_pinv_body(const ComputeMArgs& a, double* smem, JacobiShared& js,
+                                               double* diag_tmp) {
 #pragma clang fp contract(off)
-    extern __shared__ double smem[];
-    __shared__ JacobiShared js;
-    __shared__ double diag_tmp[128];
     const int R = a.R, R2 = R * R;
     const int LD = R + 1;
     const JacobiStore st = jacobi_store(smem, a.work, R);
@@ -333,14 +331,27 @@ __global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_eigen_modes(ComputeMArgs a) {
-#pragma clang fp contract(off)
+__global__ void __launch_bounds__(BS) k_gram_pinv(ComputeMArgs a) {
     extern __shared__ double smem[];
     __shared__ JacobiShared js;
     __shared__ double diag_tmp[128];
-    __shared__ uint8_t pa[8256], pb[8256];
-    const int t = blockIdx.x;  // position in the sampled-mode list
-    const int k = a.modes[t];
+    gram_pinv_body<BS>(a, smem, js, diag_tmp);
+}
+
+// computeM part 1 of several draws (the same R): block b takes args[b].
+template <int BS>
+__global__ void __launch_bounds__(BS) k_gram_pinv_batched(const ComputeMArgs* __restrict__ args) {
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double diag_tmp[128];
+    gram_pinv_body<BS>(args[blockIdx.x], smem, js, diag_tmp);
+}
+
+template <int BS>
+__device__ __forceinline__ void eigen_modes_body(const ComputeMArgs& a, int t, double* smem, JacobiShared& js,
+                                                 double* diag_tmp, uint8_t* pa, uint8_t* pb) {
+#pragma clang fp contract(off)
+    const int k = a.modes[t];  // t: position in the sampled-mode list
     const int R = a.R, R2 = R * R;
     const int LD = R + 1;
     const JacobiStore st = jacobi_store(smem, a.work + (uint64_t)(t + 1) * 3 * LD * R, R);
@@ -413,6 +424,28 @@ __global__ void __launch_bounds__(BS) k_eigen_modes(ComputeMArgs a) {
         const uint32_t node = w / ts, e = w - node * ts;
         if (e < T) tree[w] = tree[w] * Gk[e];
     }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_eigen_modes(ComputeMArgs a) {
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double diag_tmp[128];
+    __shared__ uint8_t pa[8256], pb[8256];
+    eigen_modes_body<BS>(a, (int)blockIdx.x, smem, js, diag_tmp, pa, pb);
+}
+
+// computeM part 2 of several draws (the same R): block b takes sampled-mode
+// position blk_pos[b] of args[blk_req[b]].
+template <int BS>
+__global__ void __launch_bounds__(BS) k_eigen_modes_batched(const ComputeMArgs* __restrict__ args,
+                                                            const uint32_t* __restrict__ blk_req,
+                                                            const uint32_t* __restrict__ blk_pos) {
+    extern __shared__ double smem[];
+    __shared__ JacobiShared js;
+    __shared__ double diag_tmp[128];
+    __shared__ uint8_t pa[8256], pb[8256];
+    eigen_modes_body<BS>(args[blk_req[blockIdx.x]], (int)blk_pos[blockIdx.x], smem, js, diag_tmp, pa, pb);
 }
 
 // ---------------------------------------------------------------------------
@@ -740,10 +773,114 @@ __device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint
     return res;
 }
 
+// One wave item of the fused draw: the GPW = 64 / SG samples
+// item * GPW + (lane / SG) of the draw described by `a` (every sampled mode
+// in [a.mi_begin, a.mi_end), then the weight when the launch ends the draw).
+// xs / qs: the group's x and leaf-scan buffers in LDS; pair_tab: the packed
+// (row, column) table of a.R.
+template <int RMAX, int SG, bool USE32>
+__device__ __forceinline__ void krp_draw_item(const DrawArgs& a, uint64_t item, double* xs, double* qs,
+                                              const uint16_t* pair_tab, int lane, int g) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
+    constexpr int GPW = 64 / SG;  // groups per wave
+    const int R = a.R;
+    const int T = (int)tri_size(R);
+    constexpr bool fp32 = USE32;
+    const uint64_t i = item * GPW + lane / SG;
+    const bool active = i < a.J;
+    double h[HPL];
+#pragma unroll
+    for (int k = 0; k < HPL; k++) h[k] = 1.0;
+    if (a.mi_begin > 0 && active) {
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) h[q] = a.h_state[i * R + g + SG * q];
+    }
+
+    for (int mi = a.mi_begin; mi < a.mi_end; mi++) {
+        const int k = a.order[mi];
+        const ModeDev& md = a.mode[k];
+        // ---- eigen stage: x = h -----------------------------------------
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = h[q];
+        wave_sync_lds();
+        double c64[EPL];
+        make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
+        double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i] : philox_uniform(a.seed, a.call, i, 2 * k))
+                           : 0.5;
+        double low, high;
+        const uint64_t ce = descend<RMAX, SG, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, g, low, high);
+        const uint64_t r = leaf_index(a.esh, ce);
+        wave_sync_lds();
+        // scaled_h = h (.) E_k[r]
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) xs[g + SG * q] = h[q] * md.E[r * R + g + SG * q];
+        wave_sync_lds();
+        // ---- Gram stage: x = scaled_h -------------------------------------
+        make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
+        const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
+                                            : philox_uniform(a.seed, a.call, i, 2 * k + 1))
+                                 : 0.5;
+        const uint64_t c = descend<RMAX, SG, fp32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, g, low, high);
+        const uint64_t idx = leaf_index(md.sh, c);
+        uint32_t res = 0;
+        if (md.sh.F > 1) {
+            const double frac = (ug - low) / (high - low);
+            const uint64_t r0 = idx * md.sh.F;
+            const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
+            if (fp32 && a.leaf32)
+                res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
+            else
+                res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
+        }
+        const uint64_t srow = idx * md.sh.F + res;
+#pragma unroll
+        for (int q = 0; q < HPL; q++)
+            if (g + SG * q < R) h[q] *= md.U64[srow * R + g + SG * q];
+        if (active && g == 0) a.samples[(uint64_t)k * a.J + i] = srow;
+        wave_sync_lds();
+    }
+    if (a.mi_end < a.m) {  // the next launch continues from h
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) a.h_state[i * R + g + SG * q] = h[q];
+        }
+        return;
+    }
+    // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
+#pragma unroll
+    for (int q = 0; q < HPL; q++)
+        if (g + SG * q < R) xs[g + SG * q] = h[q];
+    wave_sync_lds();
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < HPL; q++) {
+        const int ra = g + SG * q;
+        if (ra < R) {
+            double y = 0.0;
+            for (int b = 0; b < R; b++) y += a.pinv[ra * R + b] * xs[b];
+            part += h[q] * y;
+        }
+    }
+    const double lev = group_sum<SG>(part);
+    if (active) {
+        if (a.w_out && g == 0) a.w_out[i] = (double)R / (lev * (double)a.J);
+        if (a.h_out) {
+#pragma unroll
+            for (int q = 0; q < HPL; q++)
+                if (g + SG * q < R) a.h_out[i * R + g + SG * q] = h[q];
+        }
+    }
+    wave_sync_lds();
+}
+
 template <int RMAX, int SG, int BS, bool USE32>
 __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
     using Cfg = DrawCfg<RMAX, SG>;
-    constexpr int EPL = Cfg::EPL, HPL = Cfg::HPL;
     constexpr int GPW = 64 / SG;  // groups per wave
     __shared__ double lds_x[BS / SG][RMAX];
     __shared__ double lds_q[BS / SG][RMAX];
@@ -755,111 +892,51 @@ __global__ void __launch_bounds__(BS) k_krp_draw(DrawArgs a) {
     const int grp_in_block = threadIdx.x / SG;
     double* xs = lds_x[grp_in_block];
     double* qs = lds_q[grp_in_block];
-    const int R = a.R;
-    const int T = (int)tri_size(R);
-    constexpr bool fp32 = USE32;
     // entries R..RMAX-1 of x stay zero (row_dot's generic path reads them)
     for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
     wave_sync_lds();
-
     const uint64_t waves_total = (uint64_t)gridDim.x * (BS / 64);
     const uint64_t wave_id = (uint64_t)blockIdx.x * (BS / 64) + wave;
-    const uint64_t per_wave = GPW;
-    const uint64_t n_items = (a.J + per_wave - 1) / per_wave;
+    const uint64_t n_items = (a.J + GPW - 1) / GPW;
+    for (uint64_t item = wave_id; item < n_items; item += waves_total)
+        krp_draw_item<RMAX, SG, USE32>(a, item, xs, qs, pair_tab, lane, g);
+}
 
+// Packed draw of several requests (the same R and precision) in one launch:
+// request r owns the wave items [item_off[r], item_off[r + 1]) (one item per
+// GPW of its samples); a wave takes items of the whole launch in turn and
+// draws them with its request's arguments (fused: every sampled mode, then
+// the weight).  Each request's samples are those of its own k_krp_draw.
+template <int RMAX, int SG, int BS, bool USE32>
+__global__ void __launch_bounds__(BS) k_krp_draw_packed(const DrawArgs* __restrict__ reqs,
+                                                        const uint64_t* __restrict__ item_off, uint32_t n_req,
+                                                        int R) {
+    using Cfg = DrawCfg<RMAX, SG>;
+    __shared__ double lds_x[BS / SG][RMAX];
+    __shared__ double lds_q[BS / SG][RMAX];
+    __shared__ uint16_t pair_tab[Cfg::TMAX];
+    fill_pairs<BS>(pair_tab, R);
+    const int lane = threadIdx.x & 63;
+    const int g = lane % SG;
+    const int wave = threadIdx.x / 64;
+    const int grp_in_block = threadIdx.x / SG;
+    double* xs = lds_x[grp_in_block];
+    double* qs = lds_q[grp_in_block];
+    for (int b = g; b < RMAX; b += SG) xs[b] = 0.0;
+    wave_sync_lds();
+    const uint64_t waves_total = (uint64_t)gridDim.x * (BS / 64);
+    const uint64_t wave_id = (uint64_t)blockIdx.x * (BS / 64) + wave;
+    const uint64_t n_items = item_off[n_req];
     for (uint64_t item = wave_id; item < n_items; item += waves_total) {
-        const uint64_t i = item * per_wave + lane / SG;
-        const bool active = i < a.J;
-        double h[HPL];
-#pragma unroll
-        for (int k = 0; k < HPL; k++) h[k] = 1.0;
-        if (a.mi_begin > 0 && active) {
-#pragma unroll
-            for (int q = 0; q < HPL; q++)
-                if (g + SG * q < R) h[q] = a.h_state[i * R + g + SG * q];
+        // request of this item: the last r with item_off[r] <= item
+        uint32_t lo = 0, hi = n_req;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (item_off[mid] <= item) lo = mid;
+            else hi = mid;
         }
-
-        for (int mi = a.mi_begin; mi < a.mi_end; mi++) {
-            const int k = a.order[mi];
-            const ModeDev& md = a.mode[k];
-            // ---- eigen stage: x = h -----------------------------------------
-#pragma unroll
-            for (int q = 0; q < HPL; q++)
-                if (g + SG * q < R) xs[g + SG * q] = h[q];
-            wave_sync_lds();
-            double c64[EPL];
-            make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
-            double ue = active ? (a.draws ? a.draws[(uint64_t)k * a.J + i]
-                                          : philox_uniform(a.seed, a.call, i, 2 * k))
-                               : 0.5;
-            double low, high;
-            const uint64_t ce =
-                descend<RMAX, SG, false>(md, a.esh, a.ts64, a.ts32, c64, md.etree, ue, g, low, high);
-            const uint64_t r = leaf_index(a.esh, ce);
-            wave_sync_lds();
-            // scaled_h = h (.) E_k[r]
-#pragma unroll
-            for (int q = 0; q < HPL; q++)
-                if (g + SG * q < R) xs[g + SG * q] = h[q] * md.E[r * R + g + SG * q];
-            wave_sync_lds();
-            // ---- Gram stage: x = scaled_h -------------------------------------
-            make_coefs<RMAX, SG>(xs, pair_tab, T, g, c64);
-            const double ug = active ? (a.draws ? a.draws[(uint64_t)a.N * a.J + (uint64_t)k * a.J + i]
-                                                : philox_uniform(a.seed, a.call, i, 2 * k + 1))
-                                     : 0.5;
-            const uint64_t c = descend<RMAX, SG, fp32>(md, md.sh, a.ts64, a.ts32, c64, md.tree64, ug, g, low, high);
-            const uint64_t idx = leaf_index(md.sh, c);
-            uint32_t res = 0;
-            if (md.sh.F > 1) {
-                const double frac = (ug - low) / (high - low);
-                const uint64_t r0 = idx * md.sh.F;
-                const uint32_t rows = (uint32_t)min<uint64_t>(md.sh.F, md.sh.n - r0);
-                if (fp32 && a.leaf32)
-                    res = leaf_scan<float, RMAX, SG>(md.U32, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g, lane);
-                else
-                    res = leaf_scan<double, RMAX, SG>(md.U64, r0, rows, (uint32_t)md.sh.F, R, xs, qs, frac, g,
-                                                      lane);
-            }
-            const uint64_t srow = idx * md.sh.F + res;
-#pragma unroll
-            for (int q = 0; q < HPL; q++)
-                if (g + SG * q < R) h[q] *= md.U64[srow * R + g + SG * q];
-            if (active && g == 0) a.samples[(uint64_t)k * a.J + i] = srow;
-            wave_sync_lds();
-        }
-        if (a.mi_end < a.m) {  // the next launch continues from h
-            if (active) {
-#pragma unroll
-                for (int q = 0; q < HPL; q++)
-                    if (g + SG * q < R) a.h_state[i * R + g + SG * q] = h[q];
-            }
-            continue;
-        }
-        // ---- weight: R / (J * h^T pinv h) (sampler.hpp:94-122) --------------
-#pragma unroll
-        for (int q = 0; q < HPL; q++)
-            if (g + SG * q < R) xs[g + SG * q] = h[q];
-        wave_sync_lds();
-        double part = 0.0;
-#pragma unroll
-        for (int q = 0; q < HPL; q++) {
-            const int ra = g + SG * q;
-            if (ra < R) {
-                double y = 0.0;
-                for (int b = 0; b < R; b++) y += a.pinv[ra * R + b] * xs[b];
-                part += h[q] * y;
-            }
-        }
-        const double lev = group_sum<SG>(part);
-        if (active) {
-            if (a.w_out && g == 0) a.w_out[i] = (double)R / (lev * (double)a.J);
-            if (a.h_out) {
-#pragma unroll
-                for (int q = 0; q < HPL; q++)
-                    if (g + SG * q < R) a.h_out[i * R + g + SG * q] = h[q];
-            }
-        }
-        wave_sync_lds();
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+        krp_draw_item<RMAX, SG, USE32>(reqs[r], item - item_off[r], xs, qs, pair_tab, lane, g);
     }
 }
 

@@ -3,11 +3,14 @@ libftl_hip.so) and ``serve_stream``, which replays a stream of timed
 requests against it and measures per-request latency.
 
 Requests are KRP leverage-sample draws over factor sets of uneven shape
-(N, I, R, J).  The server admits them FIFO by HBM footprint into a fixed
-device arena, runs up to ``max_active`` concurrently on separate streams and
-retires them as they finish; a request admitted while others run overlaps
-with them on the device.  Factors and uniforms come from the request's seed,
-so a result does not depend on what else is being served.
+(N, I, R, J): either over factors the server generates on the device from
+the request's seed, or over a caller's factor set registered once
+(``add_factors`` / ``add_sampler``) whose Gram trees stay resident, with
+``update_factor`` rebuilding one mode's tree.  The server admits requests
+FIFO by HBM footprint into a fixed device arena; requests whose trees are
+complete are drawn together in packed launches (one launch of each computeM
+kernel and of the draw kernel per rank and precision).  A result depends only
+on the request's factors and seed, so it is the same served alone or in a mix.
 """
 from __future__ import annotations
 
@@ -29,7 +32,7 @@ class Server:
         self._h = h
         self.max_active = int(max_active)
         self._shapes = {}
-        self._waiting_hint = 0
+        self._sets = {}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -49,18 +52,80 @@ class Server:
         _lib.check(_lib.load().ftl_server_submit(self._h, N, I.ctypes.data, int(R), int(J), jj, int(seed),
                                                  PRECISIONS[precision], ctypes.byref(rid)), "submit")
         self._shapes[rid.value] = (N, int(J), [int(d) for d in I], int(R))
-        self._waiting_hint += 1
+        return int(rid.value)
+
+    def add_factors(self, factors, J_max: int, precision: str = "fp64") -> int:
+        """Register a factor set (NumPy float64 I_k x R arrays, or torch device
+        tensors); returns its id for ``submit_on``."""
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        on_device = hasattr(factors[0], "data_ptr")
+        N = len(factors)
+        R = int(factors[0].shape[1])
+        dims = np.ascontiguousarray([int(f.shape[0]) for f in factors], dtype=np.uint64)
+        if on_device:
+            from .sampler import _dev_ptr
+            for k, f in enumerate(factors):
+                if str(f.dtype) != "torch.float64" or f.dim() != 2 or int(f.shape[1]) != R:
+                    raise TypeError(f"factors[{k}] must be a float64 I_k x R device tensor")
+            ptrs = (ctypes.c_void_p * N)(*[_dev_ptr(f, f"factors[{k}]") for k, f in enumerate(factors)])
+        else:
+            from .sampler import _require_array
+            for k, f in enumerate(factors):
+                _require_array(f, np.float64, 2, f"factors[{k}]")
+                if f.shape[1] != R:
+                    raise ValueError("factors must have R columns")
+            ptrs = (ctypes.c_void_p * N)(*[f.ctypes.data for f in factors])
+        fs = ctypes.c_uint64()
+        _lib.check(_lib.load().ftl_server_add_factors(self._h, N, dims.ctypes.data, R, ptrs, int(on_device),
+                                                      PRECISIONS[precision], int(J_max), ctypes.byref(fs)),
+                   "add_factors")
+        self._sets[int(fs.value)] = (N, [int(d) for d in dims], R)
+        return int(fs.value)
+
+    def add_sampler(self, sampler) -> int:
+        """Register the factors of an existing ``Sampler`` (J_max = its J)."""
+        fs = ctypes.c_uint64()
+        _lib.check(_lib.load().ftl_server_add_sampler(self._h, sampler.handle, ctypes.byref(fs)), "add_sampler")
+        self._sets[int(fs.value)] = (sampler.N, list(sampler.dims), sampler.R)
+        return int(fs.value)
+
+    def update_factor(self, fs: int, k: int, Uk):
+        """Replace factor k of a registered set and rebuild its tree only."""
+        on_device = hasattr(Uk, "data_ptr")
+        if on_device:
+            from .sampler import _dev_ptr
+            ptr = _dev_ptr(Uk, "Uk")
+        else:
+            from .sampler import _require_array
+            _require_array(Uk, np.float64, 2, "Uk")
+            ptr = Uk.ctypes.data
+        _lib.check(_lib.load().ftl_server_update_factor(self._h, int(fs), int(k), ptr, int(on_device)),
+                   "update_factor")
+
+    def remove_factors(self, fs: int):
+        _lib.check(_lib.load().ftl_server_remove_factors(self._h, int(fs)), "remove_factors")
+        self._sets.pop(int(fs), None)
+
+    def submit_on(self, fs: int, J: int, j=None, seed: int = 0) -> int:
+        """Queue a draw of J samples on registered factor set fs."""
+        if int(fs) not in self._sets:
+            raise ValueError(f"unknown factor set {fs}")
+        N, dims, R = self._sets[int(fs)]
+        jj = N + 1 if j is None else int(j)
+        rid = ctypes.c_uint64()
+        _lib.check(_lib.load().ftl_server_submit_on(self._h, int(fs), int(J), jj, int(seed), ctypes.byref(rid)),
+                   "submit_on")
+        self._shapes[rid.value] = (N, int(J), dims, R)
         return int(rid.value)
 
     def progress(self):
-        """Ids of the requests finished since the last call."""
-        cap = self.max_active + self._waiting_hint + 1
+        """Ids of the requests finished and not reported before."""
+        cap = max(64, len(self._shapes))
         buf = np.zeros(cap, dtype=np.uint64)
         n = ctypes.c_uint64()
         _lib.check(_lib.load().ftl_server_progress(self._h, buf.ctypes.data, cap, ctypes.byref(n)), "progress")
-        done = [int(x) for x in buf[: int(n.value)]]
-        self._waiting_hint = max(0, self._waiting_hint - len(done))
-        return done
+        return [int(x) for x in buf[: int(n.value)]]
 
     def result(self, rid: int) -> np.ndarray:
         """Samples (N x J uint64) of a finished request; releases it."""
@@ -87,14 +152,20 @@ class Server:
         keys = ["active", "waiting", "finished", "bytes_in_use", "peak_bytes", "admitted", "completed", "cancelled"]
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def wait_for(self, rid: int = 0, timeout_s: float = 60.0) -> bool:
+        """Drive the server (sleeping between polls of the device's completion
+        events) until request rid has finished -- rid 0: until some request
+        finishes that progress() has not reported -- or timeout_s passed."""
+        done = ctypes.c_int()
+        _lib.check(_lib.load().ftl_server_wait(self._h, int(rid), float(timeout_s), ctypes.byref(done)), "wait")
+        return bool(done.value)
+
     def wait(self, rid: int, timeout_s: float = 60.0) -> np.ndarray:
-        """Drive progress until request rid finishes; return its samples."""
-        t0 = time.perf_counter()
-        while True:
-            if rid in self.progress():
-                return self.result(rid)
-            if time.perf_counter() - t0 > timeout_s:
-                raise TimeoutError(f"request {rid} did not finish in {timeout_s} s")
+        """Wait until request rid finishes; return its samples."""
+        if not self.wait_for(rid, timeout_s):
+            raise TimeoutError(f"request {rid} did not finish in {timeout_s} s")
+        self.progress()  # report it (and anything else finished) as done
+        return self.result(rid)
 
 
 def random_requests(n: int, seed: int = 0):
@@ -133,6 +204,9 @@ def serve_stream(server: Server, requests, rate: float, seed: int = 0, precision
             rid = server.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)
             pending[rid] = nxt
             nxt += 1
+        # sleep in the server until a request finishes or the next one arrives
+        until_next = (arrivals[nxt] - (time.perf_counter() - t0)) if nxt < len(requests) else 0.05
+        server.wait_for(0, max(0.0, min(until_next, 0.05)))
         for rid in server.progress():
             i = pending.pop(rid)
             lat[i] = (time.perf_counter() - t0) - arrivals[i]

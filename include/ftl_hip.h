@@ -217,6 +217,18 @@ int ftl_sparse_create_device(uint32_t N, uint64_t nnz, const uint32_t* idx_dev, 
 
 int ftl_sparse_destroy(ftl_sparse* X);
 
+/*
+ * Lookup method of the tensor's probes: 0 = binary search of each mode's
+ * sorted keys (SortIdxLookup, sort_lookup.hpp:33-132), 1 = a device
+ * open-addressing coordinate hash per mode (linear probing, load factor
+ * <= 1/2) mapping each distinct key to its run of the sorted nonzeros -- the
+ * reference's HashIdxLookup (hash_lookup.hpp:13-162) in device form.  Both
+ * find the same matches in the same order.  The hash is built on the first
+ * call (needs per-mode keys of fewer than 64 bits: FTL_EUNSUPPORTED
+ * otherwise).  SparseTensor(idx, values, "hash") selects 1.
+ */
+int ftl_sparse_set_lookup(ftl_sparse* X, int method);
+
 /* out[0] = N, out[1] = nnz, out[2 + k] = size of mode k. */
 int ftl_sparse_info(ftl_sparse* X, uint64_t* out);
 
@@ -425,11 +437,15 @@ int ftl_lk_sampler_leverage(ftl_lk_sampler* s, uint32_t k, double* lev_host, dou
  * Draw-request server (no counterpart in the reference, whose only surface is
  * one PySampler per factor set, als_module.cpp:30-82).  Requests for J
  * samples of the KRP of a factor set (N, I[k], R) are admitted FIFO by HBM
- * footprint into a device arena of hbm_budget_bytes, run concurrently on up
- * to max_active streams, and retired as they finish.  Factors are generated
- * on the device: entries N(0,1) rounded to FP32, from a Philox stream of the
- * request's seed; the uniforms are the built-in Philox stream of the same
- * seed.  A result therefore depends only on the request, alone or in any mix.
+ * footprint into a device arena of hbm_budget_bytes, up to max_active at a
+ * time, and retired as they finish.  A request either carries its own
+ * factors, generated on the device (entries N(0,1) rounded to FP32 from a
+ * Philox stream of its seed) and built on a stream of its own, or draws from
+ * a registered factor set whose trees stay resident.  Every request whose
+ * trees are complete is drawn in a packed launch with the others of the same
+ * R and precision (R <= 64; one launch each of the two computeM kernels and
+ * of the draw kernel).  The uniforms are the Philox stream of the request's
+ * seed, so a result depends only on the request, alone or in any mix.
  * Not thread-safe: one host thread drives a server.
  * ------------------------------------------------------------------------ */
 typedef struct ftl_server ftl_server;
@@ -442,9 +458,43 @@ int ftl_server_destroy(ftl_server* s);
 int ftl_server_submit(ftl_server* s, uint32_t N, const uint64_t* I, uint64_t R, uint64_t J, uint32_t j,
                       uint64_t seed, int precision, uint64_t* id_out);
 
-/* Retire finished requests and admit waiting ones.  done_ids receives the ids
- * finished since the last call (*n_done of them; capacity >= max_active plus
- * the number waiting always suffices). */
+/* Register a caller's factor set: N factors U[k] (I[k] x R FP64, host or
+ * device pointers per on_device) are copied into the arena and their Gram
+ * trees built once (queued; host factors are read before the call returns).
+ * They stay resident until ftl_server_remove_factors; draws on the set ask
+ * for at most J_max samples.  This is the server form of Sampler(matrices,
+ * J, R, "efficient") (als_module.cpp:34-44).  FTL_ENOMEM when the set does
+ * not fit next to what the arena holds. */
+int ftl_server_add_factors(ftl_server* s, uint32_t N, const uint64_t* I, uint64_t R, const double* const* U,
+                           int on_device, int precision, uint64_t J_max, uint64_t* fs_out);
+
+/* Register the factors of an existing sampler handle (copied from its
+ * device factors; J_max = its J, its precision). */
+int ftl_server_add_sampler(ftl_server* s, ftl_sampler* sampler, uint64_t* fs_out);
+
+/* update_sampler(k) of a registered set (efficient_krp_sampler.hpp:73-75):
+ * after the set's queued draws, factor k is replaced by Uk and only tree k
+ * is rebuilt; later draws see the new factor. */
+int ftl_server_update_factor(ftl_server* s, uint64_t fs, uint32_t k, const double* Uk, int on_device);
+
+/* Release a registered set (deferred until its queued requests finish). */
+int ftl_server_remove_factors(ftl_server* s, uint64_t fs);
+
+/* Queue a draw of J <= J_max samples on a registered set, excluding mode j
+ * (j >= N: every mode), uniforms from the Philox stream of seed.  Draws of
+ * one set run one after another (they share its computeM buffers). */
+int ftl_server_submit_on(ftl_server* s, uint64_t fs, uint64_t J, uint32_t j, uint64_t seed, uint64_t* id_out);
+
+/* Drive the server until request id has finished (id 0: until any request
+ * finishes that progress() has not reported yet) or timeout_s seconds
+ * passed; *finished = 1 when it has.  The host sleeps between polls of the
+ * device's completion events instead of spinning. */
+int ftl_server_wait(ftl_server* s, uint64_t id, double timeout_s, int* finished);
+
+/* Retire finished requests, admit waiting ones and launch the packed draws
+ * of requests whose trees are complete.  done_ids receives up to capacity ids
+ * of requests finished and not reported before (*n_done of them); the rest
+ * are reported by later calls. */
 int ftl_server_progress(ftl_server* s, uint64_t* done_ids, uint64_t capacity, uint64_t* n_done);
 
 /* Samples (N x J uint64, row j zero when j < N) of a finished request, then

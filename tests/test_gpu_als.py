@@ -46,12 +46,15 @@ LOOKUP_CASES = [
 ]
 
 
+@pytest.mark.parametrize("method", ["sort", "hash"])
 @pytest.mark.parametrize("cid,dims,nnz,j,unique", LOOKUP_CASES, ids=[c[0] for c in LOOKUP_CASES])
-def test_lookup_and_mttkrp_match_oracle(ftl, oracle, cid, dims, nnz, j, unique):
-    print(f"[gpu] sparse lookup/mttkrp case={cid}", flush=True)
+def test_lookup_and_mttkrp_match_oracle(ftl, oracle, cid, dims, nnz, j, unique, method):
+    """Matched nonzero ids and counts bit-exact against the oracle's sorted
+    lookup, with the binary search and with the device coordinate hash."""
+    print(f"[gpu] sparse lookup/mttkrp case={cid} method={method}", flush=True)
     rng = np.random.default_rng(_seed(cid))
     idx, vals = random_sparse(dims, nnz, rng, unique)
-    X = ftl.SparseTensor(idx, vals, "sort")
+    X = ftl.SparseTensor(idx, vals, method)
     S = oracle.OracleSparse(idx, vals)
     assert X.dims == S.dims
     np.testing.assert_allclose(X.get_normsq(), S.normsq(), rtol=1e-13)
@@ -318,3 +321,25 @@ def test_als_prod_exact_is_monotone(ftl):
     assert all(b >= a - 1e-12 for a, b in zip(fits, fits[1:])), fits
     floor = 1 - np.linalg.norm(noise) / np.linalg.norm(vals)
     assert fits[-1] > floor - 1e-3, (fits[-1], floor)
+
+
+def test_hash_lookup_als_updates_equal_sort(ftl):
+    """Sampled ALS updates with the coordinate hash give bitwise the factors
+    and counts of the sorted lookup (same matches, same order)."""
+    rng = np.random.default_rng(44)
+    dims, R, J = (300, 200, 150), 8, 3000
+    idx, vals = _skewed_sparse(dims, 20000, rng)
+    U0 = [rng.standard_normal((d, R)) for d in dims]
+    out = {}
+    for method in ("sort", "hash"):
+        lr = ftl.LowRankTensor(R, [u.copy() for u in U0])
+        als = ftl.ALS(lr, ftl.SparseTensor(idx, vals, method))
+        als.initialize_ds_als(J, seed=9)
+        stats = []
+        for j in (0, 1, 2, 0):
+            als.execute_ds_als_update(j, True, True)
+            stats.append(als.last_stats()["found"])
+        out[method] = ([u.copy() for u in lr.U], stats)
+    assert out["sort"][1] == out["hash"][1]
+    for a, b in zip(out["sort"][0], out["hash"][0]):
+        np.testing.assert_array_equal(a, b)

@@ -400,3 +400,4 @@ def test_fp32_sampler_matrix_core_leaf_grams(ftl, oracle, R):
     b = np.zeros((3, J), dtype=np.uint64)
     s.KRPDrawSamples(4, b, draws)
     assert np.mean(np.any(a != b, axis=0)) < 1e-3
+

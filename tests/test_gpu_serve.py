@@ -123,3 +123,89 @@ def test_bad_requests(ftl):
         srv.submit([100, 100], 8, 10, precision="fp16")
     with pytest.raises(ValueError):
         srv.result(12345)
+
+
+def _one_shot(ftl, U, J, R, j, seed, precision):
+    s = ftl.Sampler(U, J, R, "efficient", precision=precision)
+    s.set_seed(seed, 0)
+    out = np.zeros((len(U), J), dtype=np.uint64)
+    s.KRPDrawSamples(len(U) + 1 if j is None else j, out)
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_registered_factor_set_equals_sampler(ftl, precision):
+    """Draws on a registered (resident) factor set equal the one-shot sampler
+    on the same factors and seed, for J below J_max and an excluded mode; an
+    update of one factor rebuilds its tree for the next draw."""
+    print(f"[gpu] registered factor set precision={precision}", flush=True)
+    rng = np.random.default_rng(21)
+    dims, R, Jmax = [30000, 4000, 9000], 16, 8192
+    U = [rng.standard_normal((d, R)) for d in dims]
+    srv = ftl.Server(8 << 30, 4)
+    fs = srv.add_factors(U, Jmax, precision)
+    for J, j, seed in ((Jmax, None, 5), (3001, 1, 6), (1, None, 7)):
+        got = srv.wait(srv.submit_on(fs, J, j, seed))
+        exp = _one_shot(ftl, U, J, R, j, seed, precision)
+        if j is not None:
+            exp[j] = 0
+        np.testing.assert_array_equal(got, exp, err_msg=f"J={J} j={j}")
+    U1 = rng.standard_normal((dims[1], R)) * rng.random((dims[1], 1))
+    srv.update_factor(fs, 1, U1)
+    got = srv.wait(srv.submit_on(fs, 2048, None, 9))
+    np.testing.assert_array_equal(got, _one_shot(ftl, [U[0], U1, U[2]], 2048, R, None, 9, precision))
+    srv.remove_factors(fs)
+    assert srv.stats()["bytes_in_use"] == 0
+
+
+def test_factor_sets_and_generated_requests_mixed(ftl):
+    """Factor-set draws (one of them registered from a Sampler handle) and
+    seed-generated requests submitted together, several per packed launch:
+    each result equals the same request served alone."""
+    print("[gpu] factor sets and generated requests mixed", flush=True)
+    rng = np.random.default_rng(22)
+    sets = []
+    for dims, R in (([5000, 6000, 7000], 32), ([2000, 3000, 4000, 5000], 8), ([100000, 900], 32)):
+        sets.append(([rng.standard_normal((d, R)) for d in dims], R))
+    srv = ftl.Server(32 << 30, 8)
+    fss = [srv.add_factors(U, 4096, "fp32") for U, _ in sets[:2]]
+    smp = ftl.Sampler(sets[2][0], 4096, sets[2][1], "efficient", precision="fp32")
+    fss.append(srv.add_sampler(smp))
+    jobs = {}
+    for rep in range(2):
+        for i, fs in enumerate(fss):
+            jobs[srv.submit_on(fs, 4096 - 1000 * rep, None, 100 + 10 * i + rep)] = ("fs", i, rep)
+        for i, q in enumerate(MIX[:4]):
+            jobs[srv.submit(q["dims"], q["R"], q["J"], None, q["seed"] + rep, "fp32")] = ("gen", i, rep)
+    got = {}
+    while len(got) < len(jobs):
+        srv.wait_for(0, 10.0)
+        for rid in srv.progress():
+            got[rid] = srv.result(rid)
+    for rid, (kind, i, rep) in jobs.items():
+        if kind == "fs":
+            U, R = sets[i]
+            exp = _one_shot(ftl, U, 4096 - 1000 * rep, R, None, 100 + 10 * i + rep, "fp32")
+        else:
+            q = MIX[i]
+            exp = _run_alone(ftl, {**q, "seed": q["seed"] + rep}, "fp32")
+        np.testing.assert_array_equal(got[rid], exp, err_msg=f"{kind} {i} rep {rep}")
+    for fs in fss:
+        srv.remove_factors(fs)
+    assert srv.stats()["bytes_in_use"] == 0
+
+
+def test_factor_set_errors(ftl):
+    srv = ftl.Server(1 << 30, 2)
+    U = [np.ones((100, 4)), np.ones((50, 4))]
+    fs = srv.add_factors(U, 100)
+    with pytest.raises(ValueError):
+        srv.submit_on(fs, 101)          # above J_max
+    with pytest.raises(ValueError):
+        srv.submit_on(fs + 7, 10)       # unknown set
+    with pytest.raises(ValueError):
+        srv.update_factor(fs, 2, np.ones((50, 4)))   # mode out of range
+    with pytest.raises(MemoryError):
+        srv.add_factors([np.ones((10 ** 7, 8))] * 3, 1000)  # over the budget
+    srv.remove_factors(fs)
