@@ -82,6 +82,7 @@ SIGNATURES = {
     "ftl_als_lowrank_normsq": ([_vp, _vp], _i32),
     "ftl_als_factor_device": ([_vp, _u32, _vp], _i32),
     "ftl_als_last_stats": ([_vp, _vp], _i32),
+    "ftl_als_last_draw": ([_vp, _vp, _vp, _vp], _i32),
     "ftl_als_enable_timing": ([_vp, _i32], _i32),
     "ftl_als_stream": ([_vp], _vp),
     # draw-request server

@@ -385,6 +385,17 @@ class ALS:
                 "mttkrp_solve_ms", "renormalize_rebuild_ms"]
         return {k: (int(v) if i < 4 else float(v)) for i, (k, v) in enumerate(zip(keys, out))}
 
+    def last_draw(self):
+        """(samples N x J uint64, h J x R, weights J) of the last sampled update."""
+        if self.J is None:
+            raise RuntimeError("initialize_ds_als has not been called")
+        N, J, R = self.cp_decomp.N, self.J, self.cp_decomp.R
+        samples = np.zeros((N, J), dtype=np.uint64)
+        h = np.empty((J, R))
+        w = np.empty(J)
+        _lib.check(_lib.load().ftl_als_last_draw(self._fh._h, _ptr(samples), _ptr(h), _ptr(w)), "last_draw")
+        return samples, h, w
+
     def factor_device_ptr(self, k: int) -> int:
         p = ctypes.c_void_p()
         _lib.check(_lib.load().ftl_als_factor_device(self._fh._h, k, ctypes.byref(p)))

@@ -559,13 +559,13 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
 // The one-block pinv kernels hold A ((R+1) R doubles) in LDS, plus V and
 // the eigenvectors when R <= 64: allow up to 132 KB of dynamic LDS.
 void pinv_lds_attrs() {
-    static bool attrs_set = false;
-    if (attrs_set) return;
-    FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      129 * 128 * 8));
-    FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      129 * 128 * 8));
-    attrs_set = true;
+    static std::once_flag once;  // thread-safe; a throw leaves it to the next call
+    std::call_once(once, [] {
+        FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<256>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
+        FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_pinv_partials<1024>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
+    });
 }
 
 template <int RMAX>
@@ -1087,6 +1087,16 @@ struct Als {
         else if (R <= 32) launch_gram<32>(Xrows, n_dev, n_host, parts, nb);
         else if (R <= 64) launch_gram<64>(Xrows, n_dev, n_host, parts, nb);
         else launch_gram<128>(Xrows, n_dev, n_host, parts, nb);
+    }
+
+    // The last sampled update's draw: samples (N x J, the excluded mode's row
+    // zero), h (J x R) and weights (J), as the sampler wrote them.
+    void last_draw(uint64_t* samples_host, double* h_host, double* w_host) {
+        FTL_REQUIRE(sampler || lk, "initialize_ds_als has not been called");
+        FTL_HIP_CHECK(hipMemcpyAsync(samples_host, samples, N * J * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        if (h_host) FTL_HIP_CHECK(hipMemcpyAsync(h_host, h, J * R * sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (w_host) FTL_HIP_CHECK(hipMemcpyAsync(w_host, w, J * sizeof(double), hipMemcpyDeviceToHost, stream));
+        FTL_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
     // execute_ds_als_update(j, renormalize, update_sampler) (als.hpp:57-220)
@@ -1880,6 +1890,13 @@ int ftl_als_last_stats(ftl_als* a, double* out8) {
     return guarded([&] {
         FTL_REQUIRE(a && out8, "null argument");
         a->impl.stats(out8);
+    });
+}
+
+int ftl_als_last_draw(ftl_als* a, uint64_t* samples_host, double* h_host, double* w_host) {
+    return guarded([&] {
+        FTL_REQUIRE(a && samples_host, "null argument");
+        a->impl.last_draw(samples_host, h_host, w_host);
     });
 }
 

@@ -121,6 +121,7 @@ struct FactorSet {
     hipStream_t stream = nullptr;  // factor copies and tree builds
     hipEvent_t ready = nullptr;    // after the last build queued
     hipEvent_t last_draw = nullptr;
+    hipEvent_t copied = nullptr;   // after the last copy of caller factors
     bool drew = false;             // last_draw recorded at least once
     uint64_t in_flight = 0;        // requests admitted and not finished
     bool busy = false;             // a draw of this set is queued and not done
@@ -272,6 +273,7 @@ struct Server {
         for (auto& b : f->blocks) arena->release(b.first, b.second);
         if (f->ready) (void)hipEventDestroy(f->ready);
         if (f->last_draw) (void)hipEventDestroy(f->last_draw);
+        if (f->copied) (void)hipEventDestroy(f->copied);
         if (f->stream) (void)hipStreamDestroy(f->stream);
         delete f;
     }
@@ -300,6 +302,7 @@ struct Server {
                                         std::to_string(arena->size() - arena->used()) + " of the budget are free"};
         FTL_HIP_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         try {
+            FTL_HIP_CHECK(hipEventCreateWithFlags(&f->copied, hipEventDisableTiming));
             for (uint32_t k = 0; k < N; k++) {
                 const size_t b = I[k] * R * sizeof(double);
                 void* p = arena->alloc(b);
@@ -309,6 +312,11 @@ struct Server {
                 FTL_HIP_CHECK(hipMemcpyAsync(p, U[k], b, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                              f->stream));
             }
+            // the caller's factors (host or device) are read before the call
+            // returns, so they may be freed or overwritten at once; the tree
+            // builds stay queued
+            FTL_HIP_CHECK(hipEventRecord(f->copied, f->stream));
+            FTL_HIP_CHECK(hipEventSynchronize(f->copied));
             f->sampler = sampler_new_async(N, f->I, Jmax, R, f->U, prec, arena.get(), f->stream);
             // the group draw kernel (the packed launch's; the same samples as
             // the one-sample-per-wave kernel a tuning row may pick)
@@ -316,8 +324,6 @@ struct Server {
             FTL_HIP_CHECK(hipEventCreateWithFlags(&f->ready, hipEventDisableTiming));
             FTL_HIP_CHECK(hipEventCreateWithFlags(&f->last_draw, hipEventDisableTiming));
             FTL_HIP_CHECK(hipEventRecord(f->ready, f->stream));
-            // host factors: the copies have read them once the stream passed them
-            if (!on_device) FTL_HIP_CHECK(hipStreamSynchronize(f->stream));
         } catch (...) {
             free_set(f.release());
             throw;
@@ -337,9 +343,12 @@ struct Server {
         if (f->drew) FTL_HIP_CHECK(hipStreamWaitEvent(f->stream, f->last_draw, 0));
         FTL_HIP_CHECK(hipMemcpyAsync(f->U[k], Uk, f->I[k] * f->R * sizeof(double),
                                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, f->stream));
+        // Uk is read before the call returns (after the set's queued draws,
+        // which the copy waits for); the rebuild stays queued
+        FTL_HIP_CHECK(hipEventRecord(f->copied, f->stream));
         sampler_rebuild(f->sampler, k, f->stream);
         FTL_HIP_CHECK(hipEventRecord(f->ready, f->stream));
-        if (!on_device) FTL_HIP_CHECK(hipStreamSynchronize(f->stream));
+        FTL_HIP_CHECK(hipEventSynchronize(f->copied));
     }
 
     void remove_factors(uint64_t fs) {

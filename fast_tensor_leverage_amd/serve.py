@@ -186,7 +186,9 @@ def serve_stream(server: Server, requests, rate: float, seed: int = 0, precision
                  keep_results: bool = False):
     """Replay requests with Poisson arrivals at `rate` per second: each is
     submitted at its arrival time and its latency is completion - arrival,
-    completion being the first progress() call that reports it.  Returns
+    completion being the first progress() call that reports it.  A request
+    with key "fs" draws J samples on that registered factor set
+    (``submit_on``); the others bring their own shapes ("dims", "R").  Returns
     latencies (s), the makespan (s) and the results when asked."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
@@ -201,7 +203,10 @@ def serve_stream(server: Server, requests, rate: float, seed: int = 0, precision
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrivals[nxt] <= now:
             q = requests[nxt]
-            rid = server.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)
+            if "fs" in q:
+                rid = server.submit_on(q["fs"], q["J"], None, q["seed"])
+            else:
+                rid = server.submit(q["dims"], q["R"], q["J"], None, q["seed"], precision)
             pending[rid] = nxt
             nxt += 1
         # sleep in the server until a request finishes or the next one arrives

@@ -368,6 +368,11 @@ int ftl_als_factor_device(ftl_als* a, uint32_t k, double** out);
 int ftl_als_last_stats(ftl_als* a, double* out8);
 int ftl_als_enable_timing(ftl_als* a, int enable);
 
+/* The draw of the last sampled update (step 1 of execute_ds_als_update,
+ * als.hpp:70-74): samples (N x J uint64, the excluded mode's row 0), h
+ * (J x R, or NULL) and weights (J, or NULL), copied to the host. */
+int ftl_als_last_draw(ftl_als* a, uint64_t* samples_host, double* h_host, double* w_host);
+
 /* Stream of the handle (hipStream_t). */
 void* ftl_als_stream(ftl_als* a);
 
@@ -460,7 +465,8 @@ int ftl_server_submit(ftl_server* s, uint32_t N, const uint64_t* I, uint64_t R, 
 
 /* Register a caller's factor set: N factors U[k] (I[k] x R FP64, host or
  * device pointers per on_device) are copied into the arena and their Gram
- * trees built once (queued; host factors are read before the call returns).
+ * trees built once (the builds are queued; the factors, host or device, are
+ * read before the call returns and may be freed at once).
  * They stay resident until ftl_server_remove_factors; draws on the set ask
  * for at most J_max samples.  This is the server form of Sampler(matrices,
  * J, R, "efficient") (als_module.cpp:34-44).  FTL_ENOMEM when the set does
@@ -474,7 +480,9 @@ int ftl_server_add_sampler(ftl_server* s, ftl_sampler* sampler, uint64_t* fs_out
 
 /* update_sampler(k) of a registered set (efficient_krp_sampler.hpp:73-75):
  * after the set's queued draws, factor k is replaced by Uk and only tree k
- * is rebuilt; later draws see the new factor. */
+ * is rebuilt; later draws see the new factor.  Uk is read before the call
+ * returns (which therefore waits for the set's queued draws); the rebuild
+ * stays queued. */
 int ftl_server_update_factor(ftl_server* s, uint64_t fs, uint32_t k, const double* Uk, int on_device);
 
 /* Release a registered set (deferred until its queued requests finish). */

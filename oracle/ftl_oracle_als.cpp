@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -111,6 +112,38 @@ double chain_prod_sum(const double* const* U, const uint64_t* dims, uint64_t N, 
 
 }  // namespace
 
+// std::stable_sort's result (unique for a given order) computed in parallel:
+// P contiguous chunks stable-sorted by separate threads, then merged pairwise
+// in rounds; std::merge takes equal elements from the left (earlier) chunk
+// first, so the merged order is the stable order.
+template <class Cmp>
+static void stable_sort_par(std::vector<uint64_t>& v, Cmp cmp) {
+    const size_t n = v.size();
+    const int T = omp_get_max_threads();
+    if (n < (1u << 20) || T <= 1) {
+        std::stable_sort(v.begin(), v.end(), cmp);
+        return;
+    }
+    size_t P = 1;
+    while (P < (size_t)T) P <<= 1;
+    std::vector<size_t> b(P + 1);
+    for (size_t i = 0; i <= P; i++) b[i] = n * i / P;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)P; i++) std::stable_sort(v.begin() + b[i], v.begin() + b[i + 1], cmp);
+    std::vector<uint64_t> tmp(n);
+    uint64_t* src = v.data();
+    uint64_t* dst = tmp.data();
+    for (size_t w = 1; w < P; w <<= 1) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long i = 0; i < (long)P; i += (long)(2 * w)) {
+            const size_t lo = b[i], mid = b[std::min(i + w, P)], hi = b[std::min(i + 2 * w, P)];
+            std::merge(src + lo, src + mid, src + mid, src + hi, dst + lo, cmp);
+        }
+        std::swap(src, dst);
+    }
+    if (src != v.data()) v.swap(tmp);
+}
+
 extern "C" {
 
 // SparseTensor(idx, values, "sort") (sparse_tensor.hpp:46-77): one lookup
@@ -127,8 +160,12 @@ void* ftlo_sparse_create(const uint32_t* idx, const double* val, uint64_t nnz, u
         L.mode = j;
         L.order.resize(nnz);
         std::iota(L.order.begin(), L.order.end(), 0);
-        std::stable_sort(L.order.begin(), L.order.end(),
-                         [&](uint64_t a, uint64_t b) { return S->less(j, a, b); });
+        stable_sort_par(L.order, [&](uint64_t a, uint64_t b) { return S->less(j, a, b); });
+        if (nnz >= (1u << 24)) {
+            fprintf(stderr, "[oracle] lookup of mode %llu sorted (%llu nonzeros)\n", (unsigned long long)j,
+                    (unsigned long long)nnz);
+            fflush(stderr);
+        }
     }
     double ns = 0.0;
     for (uint64_t i = 0; i < nnz; i++) ns += val[i] * val[i];

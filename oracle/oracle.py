@@ -259,6 +259,8 @@ class OracleSampler:
         return out
 
     def compute_m(self, j):
+        """computeM for draw j: (pinv, E); pinv holds the upper triangle
+        (the lower one is zero), E the N eigen-row matrices."""
         pinv = np.empty((self.R, self.R))
         E = np.empty((self.N, self.R, self.R))
         lib().ftlo_sampler_compute_m(self._h, j, _p(pinv), _p(E))
@@ -518,6 +520,34 @@ class OracleALS:
                                    _p(draws), int(renormalize), int(update_sampler), _p(self.sigma), _p(diag))
         if rc != 0:
             raise RuntimeError("oracle ALS update failed")
+        return {"unique_samples": int(diag[0]), "found": int(diag[1])}
+
+    def update_with_samples(self, j, samples, renormalize=True, update_sampler=True):
+        """The update of execute_ds_als_update (als.hpp:76-220) on given
+        sample indices (N x J; row j ignored) instead of this sampler's draw:
+        h = the product of the sampled rows of the other factors and weights
+        R / (J h^T pinv h) with pinv from this sampler's computeM
+        (efficient_krp_sampler.hpp:90-150, 257-267), then dedup, pinv, MTTKRP,
+        solve and renormalize as in ``update``."""
+        samples = _c(samples, np.uint64)
+        N, J, R = self.N, self.J, self.R
+        if samples.shape != (N, J):
+            raise ValueError("samples must be N x J")
+        h = np.ones((J, R))
+        for k in range(N):
+            if k != j:
+                h *= self.U[k][samples[k].astype(np.int64)]
+        up, _ = self.sampler.compute_m(j)  # upper triangle of pinv
+        pinv = np.triu(up) + np.triu(up, 1).T
+        w = R / (J * np.einsum("ir,rs,is->i", h, pinv, h))
+        dims = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
+        diag = np.zeros(4)
+        rc = lib().ftlo_als_update_drawn(self.X._h, _ptrs(self.U), _p(dims), N, J, R, j, _p(samples), _p(h),
+                                         _p(w), int(renormalize), _p(self.sigma), _p(diag))
+        if rc != 0:
+            raise RuntimeError("oracle ALS update failed")
+        if update_sampler:
+            self.sampler.update_sampler(j)
         return {"unique_samples": int(diag[0]), "found": int(diag[1])}
 
     def exact_update(self, j, renormalize=True, update_sampler=True):

@@ -28,6 +28,33 @@ def test_mttkrp_matches_brute_force(oracle, dims, nnz, j, unique):
     np.testing.assert_array_equal(out, ref)   # same summation order -> bitwise equal
 
 
+@pytest.mark.parametrize("j", [0, 2])
+def test_lookup_sort_of_large_tensors_is_stable(oracle, j):
+    """Above 2^20 nonzeros the lookup sort runs in parallel chunks merged
+    pairwise; the matched ids of every sample must still be the ascending
+    positions of the nonzeros with its coordinates (the stable order), with
+    chunk boundaries falling inside runs of equal keys."""
+    rng = np.random.default_rng(11 + j)
+    dims = (6, 5, 7)
+    nnz = 3 * 2 ** 20 + 7
+    idx = np.stack([rng.integers(0, d, size=nnz) for d in dims], axis=1).astype(np.uint32)
+    vals = rng.standard_normal(nnz)
+    S = oracle.OracleSparse(idx, vals)
+    samples = np.stack([rng.integers(0, d, size=12) for d in dims], axis=1).astype(np.uint64)
+    _, found, counts, ids = S.mttkrp(samples, rng.standard_normal((12, 2)), j, dims[j], want_matches=True)
+    other = [k for k in range(3) if k != j]
+    pos = 0
+    for u in range(12):
+        mask = np.ones(nnz, dtype=bool)
+        for k in other:
+            mask &= idx[:, k] == samples[u, k]
+        ref = np.nonzero(mask)[0]
+        assert int(counts[u]) == ref.size
+        np.testing.assert_array_equal(ids[pos:pos + ref.size], ref)
+        pos += ref.size
+    assert pos == found
+
+
 def test_mttkrp_no_matches_and_normsq(oracle):
     rng = np.random.default_rng(3)
     idx = np.array([[0, 0, 0], [1, 2, 3]], dtype=np.uint32)
@@ -146,6 +173,37 @@ def test_als_update_is_sampled_least_squares(oracle, dims, R, J, j):
     als = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in U], J)
     als.update(j, draws, renormalize=False, update_sampler=False)
     np.testing.assert_allclose(als.U[j], expect, rtol=1e-8, atol=1e-10 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("dims,R,J", [((15, 12, 10), 5, 200), ((8, 9, 7, 6), 3, 150)])
+def test_update_with_injected_samples_equals_update(oracle, dims, R, J):
+    """update_with_samples on the samples the oracle's own sampler draws
+    (h and weights recomputed from the factors and computeM's pinv) gives the
+    factors and sigma of update() with the uniforms, over a sweep; a planted
+    change of one sample changes the result."""
+    rng = np.random.default_rng(R + J)
+    N = len(dims)
+    idx, vals = random_sparse(dims, int(np.prod(dims)) // 3, rng, unique=True)
+    U = [rng.standard_normal((d, R)) for d in dims]
+    S = oracle.OracleSparse(idx, vals)
+    a = oracle.OracleALS(S, [u.copy() for u in U], J)
+    b = oracle.OracleALS(S, [u.copy() for u in U], J)
+    for j in range(N):
+        draws = rng.random(2 * N * J)
+        samples = oracle.OracleSampler([u.copy() for u in b.U], J, R).draw(j, draws)["samples"]
+        a.update(j, draws)
+        b.update_with_samples(j, samples)
+        for k in range(N):
+            np.testing.assert_allclose(b.U[k], a.U[k], rtol=1e-10, atol=1e-12 * np.abs(a.U[k]).max())
+        np.testing.assert_allclose(b.sigma, a.sigma, rtol=1e-10)
+    c = oracle.OracleALS(S, [u.copy() for u in a.U], J)
+    d = oracle.OracleALS(S, [u.copy() for u in a.U], J)
+    draws = rng.random(2 * N * J)
+    samples = oracle.OracleSampler([u.copy() for u in c.U], J, R).draw(0, draws)["samples"]
+    c.update_with_samples(0, samples)
+    samples[1, 0] = (samples[1, 0] + 1) % dims[1]
+    d.update_with_samples(0, samples)
+    assert np.abs(c.U[0] - d.U[0]).max() > 1e-8
 
 
 def test_als_sweeps_raise_fit(oracle):
