@@ -140,7 +140,45 @@ struct ModeHost {
     double* tree64 = nullptr;
     float* tree32 = nullptr;
     uint64_t tree32_elems = 0;
+    // FP32 sampler whose FP32 levels are held in FP32 only: tree64 holds the
+    // n64 FP64 nodes (k_subtree_build writes both parts)
+    bool compact = false;
+    uint64_t tree64_nodes = 0;
 };
+
+// Levels below the root of a tree shape (the depth of its deepest nodes).
+inline uint32_t shape_depth(const Shape& sh) {
+    return (uint32_t)sh.lfill_level + (sh.node_count > sh.nodes_upto_lfill ? 1 : 0);
+}
+
+// LDS of one subtree build block: a slot per wave (its leaf) and one per
+// depth below d64 - 1 (a left child waiting for its sibling).
+inline size_t subtree_lds(uint64_t R, uint32_t depth, uint32_t d64, uint32_t waves) {
+    return (size_t)(waves + depth - (d64 - 1)) * tri_stride64(R) * sizeof(double);
+}
+
+constexpr size_t kSubtreeLds = 150u * 1024u;
+
+// LDS of a k_subtree_pairs block for subtrees L levels deep: a node per
+// wave (2^(L-1) waves) in the accumulator layout, and the (row, column) of
+// each packed entry.
+inline size_t pairs_lds(uint64_t R, uint32_t L) {
+    const uint64_t nb = (R + 15) / 16, nt = nb * (nb + 1) / 2;
+    return (size_t)(1ull << (L - 1)) * nt * 256 * sizeof(double) + tri_size(R) * sizeof(uint16_t);
+}
+
+// Whether an FP32 tree is held compact: FP32 levels below d64 exist, the
+// leaves go on the matrix cores (R <= 64), the lowest FP64 level has at
+// least 64 nodes (one subtree block each: smaller trees build faster level
+// by level, and their FP64 copies are small), a subtree's 2^(L-1) leaf pairs
+// fit one block (L <= 5, L <= 4 above R = 32) and both subtree kernels fit
+// the LDS budget.
+inline bool compact_fits(uint64_t R, uint32_t depth, uint32_t d64) {
+    if (R > 64 || d64 > depth || d64 < 7) return false;
+    const uint32_t L = depth - (d64 - 1);
+    if (L > (R <= 32 ? 5u : 4u)) return false;
+    return pairs_lds(R, L) <= kSubtreeLds && subtree_lds(R, depth, d64, 4) <= kSubtreeLds;
+}
 
 inline unsigned grid_for(uint64_t work, unsigned bs) {
     uint64_t g = (work + bs - 1) / bs;
@@ -163,6 +201,8 @@ struct Sampler {
     uint32_t leaf32 = 1;
     // FP32 mode: leaf Grams on the FP64 matrix cores (option "leaf_mfma")
     uint32_t leaf_mfma = 1;
+    // option "compact_fp32": FP32 samplers keep no FP64 copy of their FP32 levels
+    uint32_t compact_opt = 1;
     uint64_t tree_pad = 0;  // zeroed entries after every tree (draw kernel over-read)
     std::vector<ModeHost> modes;
     // computeM outputs and scratch.  A sampler that owns its stream keeps two
@@ -322,7 +362,7 @@ struct Sampler {
 
     void set_mode_precision(ModeHost& m) {
         m.sh = make_shape(m.I, m.F);
-        const uint32_t depth = (uint32_t)m.sh.lfill_level + (m.sh.node_count > m.sh.nodes_upto_lfill ? 1 : 0);
+        const uint32_t depth = shape_depth(m.sh);
         if (precision == FTL_FP32) {
             const uint32_t keep64 = depth + 1 > fp32_levels ? depth + 1 - fp32_levels : 0;
             m.d64 = std::max<uint32_t>(1, keep64);
@@ -331,6 +371,41 @@ struct Sampler {
         }
         const uint64_t full = m.d64 >= 63 ? ~0ull : ((1ull << m.d64) - 1);
         m.n64 = std::min<uint64_t>(full, m.sh.node_count);
+        m.compact = precision == FTL_FP32 && leaf_mfma && compact_opt && compact_fits(R, depth, m.d64);
+    }
+
+    // (Re)allocates the trees of mode k for its current split: tree64 with
+    // every node, or the FP64 levels only when compact; tree32 with the FP32
+    // levels.  The zeroed tail pad after each covers the draw kernels'
+    // over-reads.
+    void alloc_trees(ModeHost& m) {
+        if (m.tree64) dfree(m.tree64, (m.tree64_nodes * ts64 + tree_pad) * sizeof(double));
+        if (m.tree32) dfree(m.tree32, m.tree32_elems * sizeof(float));
+        m.tree64 = nullptr;
+        m.tree32 = nullptr;
+        m.tree32_elems = 0;
+        m.tree64_nodes = m.compact ? m.n64 : m.sh.node_count;
+        m.tree64 = dalloc<double>(m.tree64_nodes * ts64 + tree_pad);
+        FTL_HIP_CHECK(hipMemsetAsync(m.tree64 + m.tree64_nodes * ts64, 0, tree_pad * sizeof(double), stream));
+        if (precision == FTL_FP32 && m.sh.node_count > m.n64) {
+            const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
+            m.tree32_elems = n32 + tree_pad;
+            m.tree32 = dalloc<float>(m.tree32_elems);
+            FTL_HIP_CHECK(hipMemsetAsync(m.tree32 + n32, 0, tree_pad * sizeof(float), stream));
+        }
+    }
+
+    template <int NB>
+    void launch_subtree_pairs(uint32_t L, dim3 grid, hipStream_t st, const BuildArgs& ba, uint64_t c0, uint64_t count,
+                              uint64_t skip, uint32_t sdep, uint32_t depth) {
+        hipLaunchKernelGGL((k_subtree_pairs<NB>), grid, dim3(64u << (L - 1)), pairs_lds(R, L), st, ba, c0, count, L,
+                           skip);
+        FTL_HIP_CHECK(hipGetLastError());
+        if (skip != ~0ull) {
+            hipLaunchKernelGGL((k_subtree_build<NB, 4>), dim3(1), dim3(256), subtree_lds(R, depth, sdep + 1, 4), st, ba,
+                               skip, (uint64_t)1, sdep);
+            FTL_HIP_CHECK(hipGetLastError());
+        }
     }
 
     // Builds on a stream other than the sampler's own occur only for samplers
@@ -354,6 +429,40 @@ struct Sampler {
         ba.tree32 = f32 ? m.tree32 : nullptr;
         ba.n64 = m.n64;
         ba.ts32 = ts32;
+        auto grid_nodes = [](uint64_t n) { return dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(n, 1), 1u << 20)); };
+        if (m.compact) {
+            // every node below depth s = d64 - 1 comes from the subtree
+            // builds (FP32 only), the nodes of depth s in FP64; the complete
+            // FP64 levels above are summed as in the full build
+            const uint32_t sdep = m.d64 - 1;
+            const uint32_t depth = shape_depth(m.sh);
+            const uint32_t L = depth - sdep;
+            const uint64_t c0 = (1ull << sdep) - 1, count = 1ull << sdep;
+            // the one node of depth s whose subtree straddles a partial bottom
+            // level (its leaves at two depths), if any: built by the fold
+            const uint64_t span = 1ull << L, q = m.sh.node_count + 1;
+            uint64_t skip = ~0ull;
+            if (q % span != 0 && q / span >= 1 && q / span - 1 >= c0 && q / span - 1 < c0 + count) skip = q / span - 1;
+            if (R <= 16) launch_subtree_pairs<1>(L, grid_nodes(count), st, ba, c0, count, skip, sdep, depth);
+            else if (R <= 32) launch_subtree_pairs<2>(L, grid_nodes(count), st, ba, c0, count, skip, sdep, depth);
+            else if (R <= 48) launch_subtree_pairs<3>(L, grid_nodes(count), st, ba, c0, count, skip, sdep, depth);
+            else launch_subtree_pairs<4>(L, grid_nodes(count), st, ba, c0, count, skip, sdep, depth);
+            for (int64_t lv = (int64_t)sdep - 1; lv >= 0;) {
+                if (lv >= 2) {
+                    const uint64_t cnt = 1ull << (lv - 2);
+                    hipLaunchKernelGGL(k_level_sum3, grid_nodes(cnt), dim3(256), 0, st, m.tree64, cnt - 1, cnt, ts64,
+                                       (float*)nullptr, m.n64, ts32);
+                    lv -= 3;
+                } else {
+                    const uint64_t cnt = 1ull << lv;
+                    hipLaunchKernelGGL(k_level_sum, grid_nodes(cnt), dim3(256), 0, st, m.tree64, cnt - 1, cnt, ts64,
+                                       (float*)nullptr, m.n64, ts32);
+                    lv -= 1;
+                }
+                FTL_HIP_CHECK(hipGetLastError());
+            }
+            return;
+        }
         if (f32 && leaf_mfma && R <= 64) {
             // FP32 samplers: leaf Grams on the matrix cores, a wave per leaf
             const uint64_t lblocks = (m.sh.leaf_count + 3) / 4;
@@ -378,7 +487,6 @@ struct Sampler {
             hipLaunchKernelGGL(k_leaf_gram<256>, dim3((unsigned)lblocks), dim3(256), lds, st, ba, lpb);
         }
         FTL_HIP_CHECK(hipGetLastError());
-        auto grid_nodes = [](uint64_t n) { return dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(n, 1), 1u << 20)); };
         // internal nodes of the lfill level (parents of a partial bottom level)
         const uint64_t start = m.sh.nodes_before_lfill, end = ba.first_leaf;
         if (end > start) {
@@ -406,12 +514,21 @@ struct Sampler {
     // (Re)allocate and fill the FP32 copy of the levels below d64 of mode k.
     void make_fp32_copy(uint32_t k) {
         ModeHost& m = modes[k];
+        const bool was_compact = m.compact;
+        set_mode_precision(m);
+        if (m.compact || was_compact) {
+            // no FP64 copy of the old FP32 levels to convert from (or the new
+            // split is compact): rebuild the mode from its FP64 factor
+            wait_foreign_draws();
+            alloc_trees(m);
+            build_mode(k);
+            return;
+        }
         if (m.tree32) {
             dfree(m.tree32, m.tree32_elems * sizeof(float));
             m.tree32 = nullptr;
             m.tree32_elems = 0;
         }
-        set_mode_precision(m);
         if (precision != FTL_FP32 || m.sh.node_count <= m.n64) return;
         wait_foreign_draws();
         const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
@@ -456,9 +573,22 @@ struct Sampler {
         } else if (key == "fp32_leaf") {
             FTL_REQUIRE(value == 0 || value == 1, "fp32_leaf must be 0 or 1");
             leaf32 = (uint32_t)value;
-        } else if (key == "leaf_mfma") {
-            FTL_REQUIRE(value == 0 || value == 1, "leaf_mfma must be 0 or 1");
-            leaf_mfma = (uint32_t)value;
+        } else if (key == "leaf_mfma" || key == "compact_fp32") {
+            FTL_REQUIRE(value == 0 || value == 1, key + " must be 0 or 1");
+            (key == "leaf_mfma" ? leaf_mfma : compact_opt) = (uint32_t)value;
+            // trees held compact, or due to be, are rebuilt in the new form
+            // (none yet while the constructor applies the tuning table)
+            for (uint32_t k = 0; k < (uint32_t)modes.size(); k++) {
+                ModeHost& m = modes[k];
+                const bool was = m.compact;
+                set_mode_precision(m);
+                if (m.compact != was) {
+                    wait_foreign_draws();
+                    alloc_trees(m);
+                    build_mode(k);
+                }
+            }
+            FTL_HIP_CHECK(hipStreamSynchronize(stream));
         } else if (key == "fp32_levels") {
             FTL_REQUIRE(value >= 0 && value <= 64, "fp32_levels must be in [0, 64]");
             fp32_levels = value == 0 ? kDefaultFp32Levels : (uint32_t)value;
@@ -502,6 +632,12 @@ struct Sampler {
         if (key == "fp32_levels") return fp32_levels;
         if (key == "fp32_leaf") return leaf32;
         if (key == "leaf_mfma") return leaf_mfma;
+        if (key == "compact_fp32") return compact_opt;
+        if (key == "compact_modes") {
+            int64_t c = 0;
+            for (const auto& m : modes) c += m.compact ? 1 : 0;
+            return c;
+        }
         if (key == "fp64_levels") return modes.empty() ? 0 : modes[0].d64;
         throw Error{FTL_EINVAL, "unknown option '" + key + "'"};
     }
@@ -541,10 +677,17 @@ struct Sampler {
             FTL_HIP_CHECK(hipEventCreateWithFlags(&trees_ev, hipEventDisableTiming));
             n_msets = 2;
         }
-        static bool attrs_set = false;
-        if (!attrs_set) {
+        // once per process (thread-safe: a throw leaves it to the next call)
+        static std::once_flag attrs_once;
+        std::call_once(attrs_once, [] {
             FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_leaf_gram<256>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            const void* sb[] = {(const void*)k_subtree_build<1, 4>, (const void*)k_subtree_build<2, 4>,
+                                (const void*)k_subtree_build<3, 4>, (const void*)k_subtree_build<4, 4>,
+                                (const void*)k_subtree_pairs<1>,    (const void*)k_subtree_pairs<2>,
+                                (const void*)k_subtree_pairs<3>,    (const void*)k_subtree_pairs<4>};
+            for (const void* f : sb)
+                FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSubtreeLds));
             // computeM blocks hold A ((R+1) R doubles) in LDS, plus V and the
             // eigenvectors when R <= 64: at most 132 KB of dynamic LDS
             const void* cm[] = {(const void*)k_gram_pinv<256>, (const void*)k_eigen_modes<256>,
@@ -555,8 +698,7 @@ struct Sampler {
                                 (const void*)k_gram_pinv_batched<1024>, (const void*)k_eigen_modes_batched<1024>};
             for (const void* f : cm)
                 FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 129 * 128 * 8));
-            attrs_set = true;
-        }
+        });
         // Tail pad of every tree: the draw kernels read 2 SG ceil(Tmax/(2 SG))
         // entries from a node start (SG up to 512 lanes per sample), Tmax
         // being the packed size for the kernel's compile-time rank bound (R
@@ -581,17 +723,8 @@ struct Sampler {
                 FTL_HIP_CHECK(hipMemcpyAsync(m.U64, U[k], m.I * R * sizeof(double),
                                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
             }
-            m.tree64 = dalloc<double>(m.sh.node_count * ts64 + pad);
-            FTL_HIP_CHECK(hipMemsetAsync(m.tree64 + m.sh.node_count * ts64, 0, pad * sizeof(double), stream));
-            if (precision == FTL_FP32) {
-                m.U32 = dalloc<float>(m.I * R);
-                if (m.sh.node_count > m.n64) {
-                    const uint64_t n32 = (m.sh.node_count - m.n64) * ts32;
-                    m.tree32_elems = n32 + pad;
-                    m.tree32 = dalloc<float>(m.tree32_elems);
-                    FTL_HIP_CHECK(hipMemsetAsync(m.tree32 + n32, 0, pad * sizeof(float), stream));
-                }
-            }
+            alloc_trees(m);
+            if (precision == FTL_FP32) m.U32 = dalloc<float>(m.I * R);
             build_mode(k);
         }
         const uint64_t R2 = R * R;
@@ -645,12 +778,14 @@ struct Sampler {
             m.I = I[k];
             m.F = std::min<uint64_t>(R, I[k]);
             m.sh = make_shape(m.I, m.F);
-            const uint32_t depth = (uint32_t)m.sh.lfill_level + (m.sh.node_count > m.sh.nodes_upto_lfill ? 1 : 0);
+            const uint32_t depth = shape_depth(m.sh);
             uint32_t d64 = depth + 1;
             if (prec == FTL_FP32) d64 = std::max<uint32_t>(1, depth + 1 > fp32_lv ? depth + 1 - fp32_lv : 0);
             const uint64_t full = d64 >= 63 ? ~0ull : ((1ull << d64) - 1);
             const uint64_t n64 = std::min<uint64_t>(full, m.sh.node_count);
-            total += up((m.sh.node_count * ts64 + pad) * 8);
+            // the default options (matrix-core leaves, compact FP32 trees)
+            const bool compact = prec == FTL_FP32 && compact_fits(R, depth, d64);
+            total += up(((compact ? n64 : m.sh.node_count) * ts64 + pad) * 8);
             if (prec == FTL_FP32) {
                 total += up(m.I * R * 4);
                 if (m.sh.node_count > n64) total += up(((m.sh.node_count - n64) * ts32 + pad) * 4);
@@ -778,17 +913,19 @@ struct Sampler {
     // at once, each looping over samples i = blockIdx.x + k * gridDim.x.
     template <int RMAX, int BS, bool USE32>
     void launch_draw_block(const DrawArgs& a, hipStream_t st) {
-        static int resident = 0;  // blocks per CU, per instantiation
-        static int cus = 0;
-        if (resident == 0) {
-            int dev = 0;
+        // blocks per CU and CU count, per instantiation (a function-local
+        // static: initialised once, thread-safe; a throw leaves it unset)
+        struct Occ {
+            int resident, cus;
+        };
+        static const Occ occ = [] {
+            int dev = 0, cus = 0, nb = 0;
             FTL_HIP_CHECK(hipGetDevice(&dev));
             FTL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-            int nb = 0;
             FTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_krp_draw_block<RMAX, BS, USE32>, BS, 0));
-            resident = std::max(1, nb);
-        }
-        uint64_t blocks = std::min<uint64_t>(J, (uint64_t)resident * (uint64_t)std::max(1, cus));
+            return Occ{std::max(1, nb), std::max(1, cus)};
+        }();
+        uint64_t blocks = std::min<uint64_t>(J, (uint64_t)occ.resident * (uint64_t)occ.cus);
         if (grid_override) blocks = std::min<uint64_t>(blocks, grid_override);
         blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 20));
         hipLaunchKernelGGL((k_krp_draw_block<RMAX, BS, USE32>), dim3((unsigned)blocks), dim3(BS), 0, st, a);
@@ -996,10 +1133,19 @@ struct Sampler {
         FTL_REQUIRE(k < N, "mode index out of range");
         const ModeHost& m = modes[k];
         FTL_REQUIRE(capacity >= m.sh.node_count, "output holds fewer nodes than the tree");
+        // a compact FP32 tree: the FP64 levels, then the FP32 ones widened
         std::vector<double> packed(m.sh.node_count * ts64);
-        FTL_HIP_CHECK(hipMemcpyAsync(packed.data(), m.tree64, packed.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                     stream));
+        FTL_HIP_CHECK(hipMemcpyAsync(packed.data(), m.tree64, m.tree64_nodes * ts64 * sizeof(double),
+                                     hipMemcpyDeviceToHost, stream));
+        std::vector<float> p32;
+        if (m.tree64_nodes < m.sh.node_count) {
+            p32.resize((m.sh.node_count - m.n64) * ts32);
+            FTL_HIP_CHECK(hipMemcpyAsync(p32.data(), m.tree32, p32.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                         stream));
+        }
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        for (uint64_t c = m.tree64_nodes; c < m.sh.node_count; c++)
+            for (uint32_t e = 0; e < ts64; e++) packed[c * ts64 + e] = (double)p32[(c - m.n64) * ts32 + e];
         const uint64_t R2 = R * R;
         for (uint64_t c = 0; c < m.sh.node_count; c++) {
             double* dst = out + c * R2;
@@ -1165,7 +1311,11 @@ void sampler_rebuild(Sampler* s, uint32_t j, hipStream_t st) {
 }
 
 SamplerView sampler_view(ftl_sampler* h) {
-    const Sampler& s = h->impl;
+    Sampler& s = h->impl;
+    // the factors are settled: no update queued on the handle's stream (or
+    // a caller's draw stream) is still writing them
+    s.wait_foreign_draws();
+    FTL_HIP_CHECK(hipStreamSynchronize(s.stream));
     SamplerView v{};
     v.N = s.N;
     for (uint32_t k = 0; k < s.N; k++) {

@@ -115,22 +115,20 @@ __global__ void __launch_bounds__(BS) k_leaf_gram(BuildArgs a, uint32_t lpb) {
 // rows, the FP64 leaf and its FP32 copy like k_leaf_gram.
 typedef double lg_f64x4 __attribute__((ext_vector_type(4)));
 
+// One wave: the leaf Gram of `node` as NB (NB + 1) / 2 accumulator tiles
+// (upper blocks (p, q), p <= q, of 16 x 16, in the order p, then q), and the
+// leaf's FP32 factor rows when a.U32 is set.
 template <int NB>
-__global__ void __launch_bounds__(256) k_leaf_gram_mfma(BuildArgs a) {
+__device__ inline void leaf_mfma_tiles(const BuildArgs& a, uint64_t node, uint32_t lane,
+                                       lg_f64x4 (&acc)[NB * (NB + 1) / 2]) {
     constexpr int NT = NB * (NB + 1) / 2;
     const uint32_t R = a.R;
     const uint32_t F = (uint32_t)a.sh.F;
-    const uint32_t T = (uint32_t)tri_size(R);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= a.sh.leaf_count) return;  // whole waves; no barrier follows
-    const uint64_t node = a.first_leaf + i;
     const uint64_t r0 = leaf_index(a.sh, node) * F;
     const uint32_t rows = (uint32_t)min<uint64_t>(F, a.sh.n - r0);
     const uint32_t kr = lane >> 4, ci = lane & 15;
     const double* __restrict__ U = a.U + r0 * R;
     float* __restrict__ U32 = a.U32 ? a.U32 + r0 * R : nullptr;
-    lg_f64x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = lg_f64x4{0.0, 0.0, 0.0, 0.0};
     // four 4-row steps per trip: their 4 NB loads are issued before the MFMAs
@@ -168,7 +166,13 @@ __global__ void __launch_bounds__(256) k_leaf_gram_mfma(BuildArgs a) {
                 }
         }
     }
-    double* out = a.tree + node * (uint64_t)a.ts64;
+}
+
+// The packed entries of the tiles held by this lane: put(e, value) for each
+// upper-triangle entry e (< T) the lane holds.
+template <int NB, class Put>
+__device__ inline void leaf_mfma_put(const lg_f64x4 (&acc)[NB * (NB + 1) / 2], uint32_t lane, uint32_t R, Put put) {
+    const uint32_t kr = lane >> 4, ci = lane & 15;
     int t = 0;
 #pragma unroll
     for (int p = 0; p < NB; p++)
@@ -177,18 +181,254 @@ __global__ void __launch_bounds__(256) k_leaf_gram_mfma(BuildArgs a) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const uint32_t r = 16 * p + 4 * v + kr, c = 16 * q + ci;
-                if (r < R && c < R && r <= c) {
-                    const uint32_t e = r * R - r * (r - 1) / 2 + c - r;
-                    out[e] = acc[t][v];
-                    store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, acc[t][v]);
-                }
+                if (r < R && c < R && r <= c) put(r * R - r * (r - 1) / 2 + c - r, acc[t][v]);
             }
             t++;
         }
+}
+
+template <int NB>
+__global__ void __launch_bounds__(256) k_leaf_gram_mfma(BuildArgs a) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    const uint32_t T = (uint32_t)tri_size(a.R);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.sh.leaf_count) return;  // whole waves; no barrier follows
+    const uint64_t node = a.first_leaf + i;
+    lg_f64x4 acc[NT];
+    leaf_mfma_tiles<NB>(a, node, lane, acc);
+    double* out = a.tree + node * (uint64_t)a.ts64;
+    leaf_mfma_put<NB>(acc, lane, a.R, [&](uint32_t e, double v) {
+        out[e] = v;
+        store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, v);
+    });
     const uint32_t tsw = a.tree32 && a.ts32 > a.ts64 ? a.ts32 : a.ts64;
     for (uint32_t e = T + lane; e < tsw; e += 64) {
         if (e < a.ts64) out[e] = 0.0;
         store_f32(a.tree32, a.n64, a.ts32, a.ts64, node, e, 0.0);
+    }
+}
+
+// FP32 samplers whose FP32 levels are stored only in FP32: a block of WPB
+// waves builds the whole subtree of one node c at depth s, the lowest FP64
+// level.  The subtree's leaves are taken in order, WPB at a time (a wave
+// each, as k_leaf_gram_mfma; thread 0 lists them in LDS), staged as FP64
+// packed nodes in LDS and written FP32.  Each leaf is then folded in: a left
+// child waits in the slot of its depth; a right child is added to it (parent
+// = left + right in FP64, the additions of the level-wise build, so every
+// node equals that build's) and the parent goes on up, written FP32, until
+// c, which is written FP64.  Every thread touches the LDS entries
+// e = threadIdx.x + 64 WPB k only, except the waves' leaf stores, which
+// barriers separate from the rest.  Dynamic LDS: (WPB + depth - s) slots of
+// ts64 doubles.
+template <int NB, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_subtree_build(BuildArgs a, uint64_t c0, uint64_t count, uint32_t s) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    constexpr uint32_t BS = 64 * WPB;
+    extern __shared__ double slot[];
+    __shared__ uint64_t lv[WPB];
+    __shared__ uint64_t next_leaf;
+    __shared__ int n_batch, more_s;
+    const uint32_t ts = a.ts64, ts32 = a.ts32;
+    const uint32_t T = (uint32_t)tri_size(a.R);
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t first_leaf = a.first_leaf;
+    for (uint64_t b = blockIdx.x; b < count; b += gridDim.x) {
+        const uint64_t c = c0 + b;
+        if (c >= first_leaf) {
+            // a node of depth s that is a leaf: an FP64 node of its own
+            if (w == 0) {
+                lg_f64x4 acc[NT];
+                leaf_mfma_tiles<NB>(a, c, lane, acc);
+                double* out = a.tree + c * (uint64_t)ts;
+                leaf_mfma_put<NB>(acc, lane, a.R, [&](uint32_t e, double v) { out[e] = v; });
+                for (uint32_t e = T + lane; e < ts; e += 64) out[e] = 0.0;
+            }
+            continue;
+        }
+        if (threadIdx.x == 0) {
+            uint64_t x = c;  // the subtree's first leaf
+            while (x < first_leaf) x = 2 * x + 1;
+            next_leaf = x;
+            more_s = 1;
+        }
+        int more = 1;
+        while (more) {
+            if (threadIdx.x == 0) {
+                // up to WPB leaves in order from next_leaf
+                uint64_t x = next_leaf;
+                int nl = 0;
+                int m = 1;
+                while (m && nl < WPB) {
+                    lv[nl++] = x;
+                    uint64_t y = x;
+                    while (y != c && (y & 1) == 0) y = (y - 1) >> 1;  // up while a right child
+                    if (y == c) {
+                        m = 0;
+                    } else {
+                        y = y + 1;  // the right sibling, then down to its leftmost leaf
+                        while (y < first_leaf) y = 2 * y + 1;
+                        x = y;
+                    }
+                }
+                next_leaf = x;
+                n_batch = nl;
+                more_s = m;
+            }
+            __syncthreads();
+            const int nl = n_batch;
+            more = more_s;
+            if ((int)w < nl) {
+                lg_f64x4 acc[NT];
+                leaf_mfma_tiles<NB>(a, lv[w], lane, acc);
+                double* sl = slot + (uint64_t)w * ts;
+                leaf_mfma_put<NB>(acc, lane, a.R, [&](uint32_t e, double v) { sl[e] = v; });
+                for (uint32_t e = T + lane; e < ts; e += 64) sl[e] = 0.0;
+            }
+            __syncthreads();
+            for (int i = 0; i < nl; i++) {
+                double* cur = slot + (uint64_t)i * ts;
+                uint64_t node = lv[i];
+                float* o = a.tree32 + (node - a.n64) * (uint64_t)ts32;
+                for (uint32_t e = threadIdx.x; e < ts32; e += BS) o[e] = e < ts ? (float)cur[e] : 0.0f;
+                while (node != c) {
+                    double* pend = slot + (uint64_t)(WPB + node_depth(node) - s - 1) * ts;
+                    if (node & 1) {  // a left child waits for its sibling
+                        for (uint32_t e = threadIdx.x; e < ts; e += BS) pend[e] = cur[e];
+                        break;
+                    }
+                    node = (node - 1) >> 1;
+                    if (node == c) {
+                        double* out = a.tree + c * (uint64_t)ts;
+                        for (uint32_t e = threadIdx.x; e < ts; e += BS) out[e] = pend[e] + cur[e];
+                    } else {
+                        float* op = a.tree32 + (node - a.n64) * (uint64_t)ts32;
+                        for (uint32_t e = threadIdx.x; e < ts32; e += BS) {
+                            const double v = e < ts ? pend[e] + cur[e] : 0.0;
+                            if (e < ts) cur[e] = v;
+                            op[e] = (float)v;
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // leaf slots and the leaf list are rewritten next
+        }
+    }
+}
+
+// Compact FP32 trees, regular subtrees: node c at depth s (the lowest FP64
+// level) whose leaves all lie Lc levels below it (Lc = L or L - 1; the one
+// subtree that straddles a partial bottom level goes to k_subtree_build).  A
+// block of 2^(L-1) waves: wave w takes the sibling leaves 2w and 2w + 1, each
+// on the FP64 matrix cores as k_leaf_gram_mfma, and adds them (the tile
+// layouts match entry for entry); then at each level above, the right one of
+// two waves hands its node to the left one through its LDS slot, which adds
+// it (left + right, as the level-wise build).  Every node passes through its
+// wave's slot, from which the wave writes it in packed order (coalesced),
+// FP32 -- or FP64 for c.  Dynamic LDS: a node per wave in the accumulator
+// layout (NT x 4 x 64 doubles), then the (row, column) of each packed entry.
+template <int NB>
+__device__ inline void tiles_to_slot(const lg_f64x4 (&acc)[NB * (NB + 1) / 2], double* slot, uint32_t lane) {
+#pragma unroll
+    for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) slot[(t * 4 + v) * 64 + lane] = acc[t][v];
+}
+
+// Entries [0, ts) of a node from its wave's slot (zero from T on), FP32 or FP64.
+template <int NB, typename OutT>
+__device__ inline void slot_write(const double* slot, const uint16_t* pairs, OutT* out, uint32_t T, uint32_t ts,
+                                  uint32_t lane) {
+    for (uint32_t e = lane; e < ts; e += 64) {
+        double v = 0.0;
+        if (e < T) {
+            const uint32_t rc = pairs[e], r = rc & 0xffu, c = rc >> 8;
+            const uint32_t p = r >> 4, q = c >> 4;
+            const uint32_t t = p * NB - p * (p - 1) / 2 + (q - p);
+            v = slot[(t * 4 + ((r & 15) >> 2)) * 64 + (r & 3) * 16 + (c & 15)];
+        }
+        out[e] = (OutT)v;
+    }
+}
+
+template <int NB>
+__global__ void __launch_bounds__(NB <= 2 ? 1024 : 512)
+    k_subtree_pairs(BuildArgs a, uint64_t c0, uint64_t count, uint32_t L, uint64_t skip) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    extern __shared__ double xch[];
+    const uint32_t R = a.R, T = (uint32_t)tri_size(R);
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t waves = blockDim.x >> 6;
+    uint16_t* pairs_rc = reinterpret_cast<uint16_t*>(xch + (uint64_t)waves * NT * 256);
+    for (uint32_t r = threadIdx.x; r < R; r += blockDim.x) {
+        const uint32_t base = r * R - r * (r - 1) / 2;
+        for (uint32_t c = r; c < R; c++) pairs_rc[base + c - r] = (uint16_t)(r | (c << 8));
+    }
+    __syncthreads();
+    const uint64_t first_leaf = a.first_leaf, nodes = a.sh.node_count;
+    double* mine = xch + (uint64_t)w * NT * 256;
+    auto emit = [&](uint64_t node, bool f64) {
+        wave_sync_lds();
+        if (f64) slot_write<NB>(mine, pairs_rc, a.tree + node * (uint64_t)a.ts64, T, a.ts64, lane);
+        else slot_write<NB>(mine, pairs_rc, a.tree32 + (node - a.n64) * (uint64_t)a.ts32, T, a.ts32, lane);
+        wave_sync_lds();
+    };
+    for (uint64_t b = blockIdx.x; b < count; b += gridDim.x) {
+        const uint64_t c = c0 + b;
+        if (c == skip) continue;
+        if (c >= first_leaf) {  // a node of depth s that is a leaf: FP64
+            if (w == 0) {
+                lg_f64x4 acc[NT];
+                leaf_mfma_tiles<NB>(a, c, lane, acc);
+                tiles_to_slot<NB>(acc, mine, lane);
+                emit(c, true);
+            }
+            continue;
+        }
+        // leaves L levels below c when its first descendant there exists
+        const uint32_t Lc = (((c + 1) << L) - 1 < nodes) ? L : L - 1;
+        const uint32_t npairs = 1u << (Lc - 1);
+        lg_f64x4 A[NT];
+        uint64_t nd = 0;
+        if (w < npairs) {
+            const uint64_t la = ((c + 1) << Lc) - 1 + 2 * (uint64_t)w;
+            leaf_mfma_tiles<NB>(a, la, lane, A);
+            tiles_to_slot<NB>(A, mine, lane);
+            emit(la, false);
+            leaf_mfma_tiles<NB>(a, la + 1, lane, A);
+            // the parent: each lane adds the left leaf's entries it holds itself
+            lg_f64x4 S[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) S[t][v] = mine[(t * 4 + v) * 64 + lane] + A[t][v];
+            wave_sync_lds();
+            tiles_to_slot<NB>(A, mine, lane);
+            emit(la + 1, false);
+#pragma unroll
+            for (int t = 0; t < NT; t++) A[t] = S[t];
+            tiles_to_slot<NB>(A, mine, lane);
+            nd = (la - 1) >> 1;
+            emit(nd, nd == c);
+        }
+        for (uint32_t k = 1; k < Lc; k++) {
+            const uint32_t half = 1u << (k - 1);
+            __syncthreads();  // the right children's slots hold their nodes
+            if (w < npairs && (w & (2 * half - 1)) == 0) {  // a left child: add its sibling
+                const double* d = xch + (uint64_t)(w + half) * NT * 256;
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) A[t][v] = A[t][v] + d[(t * 4 + v) * 64 + lane];
+            }
+            __syncthreads();  // the sibling slots are read before the left children rewrite theirs
+            if (w < npairs && (w & (2 * half - 1)) == 0) {
+                tiles_to_slot<NB>(A, mine, lane);
+                nd = (nd - 1) >> 1;
+                emit(nd, nd == c);
+            }
+        }
+        __syncthreads();  // the slots are rewritten by the next subtree
     }
 }
 

@@ -37,6 +37,34 @@ def _factors(dims, R, seed):
     return [rng.standard_normal((I, R)) for I in dims]
 
 
+def _compact_expected(oracle, I, R, fp32_levels):
+    """Whether an FP32 sampler holds mode I's FP32 levels in FP32 only: it has
+    FP32 levels, R <= 64, at least 64 nodes at the lowest FP64 level, the
+    subtrees below it at most 5 levels deep (4 above R = 32), and a block's
+    LDS (a 16 x 16-tile node per leaf pair, or four leaves and one node per
+    depth) within 150 KB."""
+    sh = oracle.tree_shape(I, min(R, I))
+    depth = sh["lfill_level"] + (1 if sh["node_count"] > sh["nodes_upto_lfill"] else 0)
+    d64 = max(1, depth + 1 - fp32_levels)
+    if R > 64 or d64 > depth or d64 < 7:
+        return False
+    L = depth - (d64 - 1)
+    nb = -(-R // 16)
+    T = R * (R + 1) // 2
+    ts64 = -(-T // 16) * 16
+    return (L <= (5 if R <= 32 else 4) and 2 ** (L - 1) * nb * (nb + 1) // 2 * 2048 + 2 * T <= 150 * 1024
+            and (4 + L) * ts64 * 8 <= 150 * 1024)
+
+
+def _fp64_nodes(oracle, I, R, fp32_levels):
+    """Nodes held in FP64 by an FP32 sampler: the levels above its bottom
+    fp32_levels (at least the root)."""
+    sh = oracle.tree_shape(I, min(R, I))
+    depth = sh["lfill_level"] + (1 if sh["node_count"] > sh["nodes_upto_lfill"] else 0)
+    d64 = max(1, depth + 1 - fp32_levels)
+    return min(2 ** d64 - 1, sh["node_count"])
+
+
 @pytest.fixture(scope="module")
 def ftl():
     import fast_tensor_leverage_amd as ftl
@@ -389,7 +417,10 @@ def test_fp32_sampler_matrix_core_leaf_grams(ftl, oracle, R):
     assert s.get_option("leaf_mfma") == 1
     for k in range(3):
         t_gpu, t_ref = s.get_tree(k), ref.tree(k)
-        np.testing.assert_allclose(t_gpu, t_ref, rtol=1e-12, atol=1e-13 * np.abs(t_ref).max())
+        n64 = _fp64_nodes(oracle, dims[k], R, s.get_option("fp32_levels"))
+        # FP64 levels to rounding; the FP32 levels (held in FP32 only) to FP32 rounding
+        np.testing.assert_allclose(t_gpu[:n64], t_ref[:n64], rtol=1e-12, atol=1e-13 * np.abs(t_ref).max())
+        np.testing.assert_allclose(t_gpu[n64:], t_ref[n64:], rtol=2e-7, atol=1e-7 * np.abs(t_ref).max())
     draws = np.random.default_rng(R).random(6 * J)
     a = np.zeros((3, J), dtype=np.uint64)
     s.KRPDrawSamples(4, a, draws)
@@ -401,3 +432,63 @@ def test_fp32_sampler_matrix_core_leaf_grams(ftl, oracle, R):
     s.KRPDrawSamples(4, b, draws)
     assert np.mean(np.any(a != b, axis=0)) < 1e-3
 
+
+
+@pytest.mark.parametrize("dims,R,levels", [
+    ([70001, 40000, 300], 32, 5),             # ragged trees, the headline split; a small tree built level-wise
+    ([2 ** 18, 1024 * 80, 257], 8, 1),        # power-of-two and ragged, one FP32 level
+    ([30000, 20000, 900], 16, 2),
+    ([2 ** 17 + 3, 60000, 64], 25, 5),        # R = 25 (padded tiles)
+    ([2 ** 15 + 3, 900, 64], 25, 7),          # seven FP32 levels: too deep for one block, full build
+    ([60000, 15000, 130], 48, 4),
+    ([90000, 70000, 65], 64, 3),              # and a factor of fewer rows than two leaves' worth
+    ([9000, 700, 65], 64, 4),                 # R = 64 four levels deep: over the LDS budget, full build
+])
+def test_compact_fp32_trees_equal_the_full_build(ftl, oracle, dims, R, levels):
+    """FP32 samplers hold their FP32 levels only in FP32 (the subtree build):
+    the draws, h and weights are bitwise those of the build that also keeps
+    FP64 copies of those levels (compact_fp32 = 0), the FP64 levels are
+    bitwise equal, the FP32 levels are the FP32 rounding of the full build's
+    FP64 nodes, and the device footprint is smaller by the FP64 copies."""
+    print(f"[gpu] compact fp32 trees dims={dims} R={R} fp32_levels={levels}", flush=True)
+    U = _factors(dims, R, 7 * R + levels)
+    J = 3000
+    a = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    a.set_option("fp32_levels", levels)
+    b = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+    b.set_option("compact_fp32", 0)
+    b.set_option("fp32_levels", levels)
+    assert a.get_option("compact_modes") == sum(1 for I in dims if _compact_expected(oracle, I, R, levels))
+    assert b.get_option("compact_modes") == 0
+    for k in range(3):
+        ta, tb = a.get_tree(k), b.get_tree(k)
+        n64 = _fp64_nodes(oracle, dims[k], R, levels)
+        np.testing.assert_array_equal(ta[:n64], tb[:n64])
+        if _compact_expected(oracle, dims[k], R, levels):  # FP32 levels held in FP32 only
+            np.testing.assert_array_equal(ta[n64:], tb[n64:].astype(np.float32).astype(np.float64))
+        else:
+            np.testing.assert_array_equal(ta[n64:], tb[n64:])
+    draws = np.random.default_rng(R).random(6 * J)
+    for j in (0, 4):
+        out = []
+        for s in (a, b):
+            smp = np.zeros((3, J), dtype=np.uint64)
+            h = np.zeros((J, R))
+            w = np.zeros(J)
+            s.KRPDrawSamples_materialize(j, smp, h, w, draws)
+            out.append((smp, h, w))
+        for x, y in zip(out[0], out[1]):
+            np.testing.assert_array_equal(x, y)
+    # an update goes through the same subtree build
+    Un = U[1] * 0.5 + 0.1
+    for s in (a, b):
+        s.update_sampler(1, Un)
+    np.testing.assert_array_equal(a.get_tree(1)[:_fp64_nodes(oracle, dims[1], R, levels)],
+                                  b.get_tree(1)[:_fp64_nodes(oracle, dims[1], R, levels)])
+    sa, sb = np.zeros((3, J), dtype=np.uint64), np.zeros((3, J), dtype=np.uint64)
+    a.KRPDrawSamples(4, sa, draws)
+    b.KRPDrawSamples(4, sb, draws)
+    np.testing.assert_array_equal(sa, sb)
+    saved = sum(8 * (oracle.tree_shape(I, min(R, I))["node_count"] - _fp64_nodes(oracle, I, R, levels))
+                * (-(-(R * (R + 1) // 2) // 16) * 16) for I in dims if _compact_expected(oracle, I, R, levels))
+    assert b.info()["device_bytes"] - a.info()["device_bytes"] >= saved - 4096
