@@ -402,8 +402,26 @@ struct Sampler {
                            skip);
         FTL_HIP_CHECK(hipGetLastError());
         if (skip != ~0ull) {
-            hipLaunchKernelGGL((k_subtree_build<NB, 4>), dim3(1), dim3(256), subtree_lds(R, depth, sdep + 1, 4), st, ba,
-                               skip, (uint64_t)1, sdep);
+            // one block: as many waves (leaves per batch) as its slots allow
+            // (the wider blocks only up to R = 32 / 48, whose registers fit them)
+            const size_t l16 = subtree_lds(R, depth, sdep + 1, 16), l8 = subtree_lds(R, depth, sdep + 1, 8);
+            bool done = false;
+            if constexpr (NB <= 2) {
+                if (l16 <= kSubtreeLds) {
+                    hipLaunchKernelGGL((k_subtree_build<NB, 16>), dim3(1), dim3(1024), l16, st, ba, skip, (uint64_t)1,
+                                       sdep);
+                    done = true;
+                }
+            }
+            if constexpr (NB <= 3) {
+                if (!done && l8 <= kSubtreeLds) {
+                    hipLaunchKernelGGL((k_subtree_build<NB, 8>), dim3(1), dim3(512), l8, st, ba, skip, (uint64_t)1, sdep);
+                    done = true;
+                }
+            }
+            if (!done)
+                hipLaunchKernelGGL((k_subtree_build<NB, 4>), dim3(1), dim3(256), subtree_lds(R, depth, sdep + 1, 4), st,
+                                   ba, skip, (uint64_t)1, sdep);
             FTL_HIP_CHECK(hipGetLastError());
         }
     }
@@ -682,10 +700,13 @@ struct Sampler {
         std::call_once(attrs_once, [] {
             FTL_HIP_CHECK(hipFuncSetAttribute((const void*)k_leaf_gram<256>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            const void* sb[] = {(const void*)k_subtree_build<1, 4>, (const void*)k_subtree_build<2, 4>,
-                                (const void*)k_subtree_build<3, 4>, (const void*)k_subtree_build<4, 4>,
-                                (const void*)k_subtree_pairs<1>,    (const void*)k_subtree_pairs<2>,
-                                (const void*)k_subtree_pairs<3>,    (const void*)k_subtree_pairs<4>};
+            const void* sb[] = {(const void*)k_subtree_build<1, 4>,  (const void*)k_subtree_build<2, 4>,
+                                (const void*)k_subtree_build<3, 4>,  (const void*)k_subtree_build<4, 4>,
+                                (const void*)k_subtree_build<1, 8>,  (const void*)k_subtree_build<2, 8>,
+                                (const void*)k_subtree_build<3, 8>,
+                                (const void*)k_subtree_build<1, 16>, (const void*)k_subtree_build<2, 16>,
+                                (const void*)k_subtree_pairs<1>,     (const void*)k_subtree_pairs<2>,
+                                (const void*)k_subtree_pairs<3>,     (const void*)k_subtree_pairs<4>};
             for (const void* f : sb)
                 FTL_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSubtreeLds));
             // computeM blocks hold A ((R+1) R doubles) in LDS, plus V and the
