@@ -134,6 +134,12 @@ def als_main(args):
                                           ctypes.byref(h)))
     t_build = time.perf_counter() - t0
     log(f"sparse tensor built in {t_build:.2f} s")
+    t_hash = None
+    if args.als_lookup == "hash":
+        t0 = time.perf_counter()
+        _lib.check(L.ftl_sparse_set_lookup(h, 1), "set_lookup")
+        t_hash = time.perf_counter() - t0
+        log(f"coordinate hash of every mode built in {t_hash:.3f} s")
     X = ftl.SparseTensor.__new__(ftl.SparseTensor)
     X._h, X.N, X.nnz, X.dims, X.last_found = h, N, nnz, list(dims), 0
     host_idx = host_vals = None
@@ -195,7 +201,7 @@ def als_main(args):
         "dtype": "f64" if args.als_precision == "fp64" else "f64 (FP32 sampler trees)",
         "data": "synthetic: Zipf(1.1) coordinates, rank-25 CP values + 1% noise, generated on the device",
         "config": {"workload": f"ALS {args.als_size}: dims={list(dims)} nnz={nnz} R={R} J={J}",
-                   "sweeps": sweeps, "precision": args.als_precision},
+                   "sweeps": sweeps, "precision": args.als_precision, "lookup": args.als_lookup},
         "update_ms": {"mean": float(upd.mean()), "median": float(np.median(upd)), "max": float(upd.max())},
         "phase_ms_mean": ph,
         "unique_samples_mean": float(np.mean([s["unique_samples"] for s in stats])),
@@ -205,6 +211,7 @@ def als_main(args):
         "fit_eval_s": fit_s,
         "wall_s_updates_and_fits": t_total,
         "sparse_construction_s": t_build,
+        "hash_construction_s": t_hash,
         "generation_s": t_gen,
     }
     if args.exact_sweeps:
@@ -627,6 +634,8 @@ def main():
     ap.add_argument("--als-size", choices=sorted(ALS_SIZES), default="c5")
     ap.add_argument("--als-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--als-oracle", action="store_true", help="rerun the sweeps on the CPU oracle and compare fits")
+    ap.add_argument("--als-lookup", choices=["sort", "hash"], default="sort",
+                    help="probe the sorted keys by binary search, or the per-mode coordinate hash")
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--exact-sweeps", type=int, default=0,
                     help="ALS workload: exact (unsampled) update sweeps after the sampled ones")

@@ -484,13 +484,25 @@ void reduce_heavy(const uint32_t* piece_off, uint32_t* counters, uint32_t* list,
     FTL_LAUNCH_CHECK();
 }
 
+// ldl: the stride of the lhs rows; an even stride from a 16-byte aligned
+// base (16-byte aligned rows) lets the pieces read column pairs
+// (k_item_partial_pairs, R <= 64).
 template <int G, int CPL>
 unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
-                       const double* pinv, double* out, double* colsq, hipStream_t st) {
+                       uint32_t ldl, const double* pinv, double* out, double* colsq, hipStream_t st) {
     constexpr int GPB = 256 / G;
     const unsigned blocks = grid_for(T, GPB, 16384);
-    hipLaunchKernelGGL((k_item_partial<G, CPL>), dim3(blocks), dim3(256), 0, st, m.seg_start, m.piece_off,
-                       m.piece_seg, m.counters, m.spay, X.val[j], lhs, R, m.partial);
+    if (ldl % 2 == 0 && R <= 64 && ((uintptr_t)lhs & 15) == 0) {
+        if (R <= 32)
+            hipLaunchKernelGGL((k_item_partial_pairs<16>), dim3(grid_for(T, 16, 16384)), dim3(256), 0, st, m.seg_start,
+                               m.piece_off, m.piece_seg, m.counters, m.spay, X.val[j], lhs, R, ldl, m.partial);
+        else
+            hipLaunchKernelGGL((k_item_partial_pairs<32>), dim3(grid_for(T, 8, 16384)), dim3(256), 0, st, m.seg_start,
+                               m.piece_off, m.piece_seg, m.counters, m.spay, X.val[j], lhs, R, ldl, m.partial);
+    } else {
+        hipLaunchKernelGGL((k_item_partial<G, CPL>), dim3(blocks), dim3(256), 0, st, m.seg_start, m.piece_off,
+                           m.piece_seg, m.counters, m.spay, X.val[j], lhs, R, m.partial);
+    }
     FTL_LAUNCH_CHECK();
     reduce_heavy(m.piece_off, m.counters, m.heavy, std::min<uint64_t>(T, X.dims[j]), R, m.partial, st);
     const unsigned fblocks = grid_for(T, GPB, kSegBlocksMax);
@@ -508,8 +520,8 @@ unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, c
 // sort them by output row (stable), form the row segments and their pieces,
 // reduce, and write each touched row of out (times pinv when given).
 unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
-                        const double* pinv, double* out, double* colsq, Matches& m, Arena& ar, CubTemp& cub,
-                        hipStream_t st) {
+                        uint32_t ldl, const double* pinv, double* out, double* colsq, Matches& m, Arena& ar,
+                        CubTemp& cub, hipStream_t st) {
     m.ensure_t(ar, T, R);
     // owner of every match (flags and segid are free until the row sort)
     FTL_HIP_CHECK(hipMemsetAsync(m.flags, 0, T * sizeof(uint32_t), st));
@@ -545,9 +557,9 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
                        m.piece_mark);
     FTL_LAUNCH_CHECK();
     inclusive_max(cub, m.piece_mark, m.piece_seg, wcap, st);
-    if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
-    if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, pinv, out, colsq, st);
-    return launch_reduce<64, 2>(X, j, m, T, lhs, R, pinv, out, colsq, st);
+    if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
+    if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
+    return launch_reduce<64, 2>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,7 +1070,7 @@ struct Als {
         ufirst = mem.alloc<uint32_t>(J);
         ustart = mem.alloc<uint32_t>(J + 1);
         hs = mem.alloc<double>(J * R);
-        hdd = mem.alloc<double>(J * R);
+        hdd = mem.alloc<double>(J * (R + (R & 1)));  // rows of even stride (a zero pad column for odd R)
         gpart = mem.alloc<double>((uint64_t)kGramBlocks * (R * (R + 1) / 2));
         pinv = mem.alloc<double>(R * R);
         work = mem.alloc<double>(3 * (R + 1) * R);
@@ -1148,7 +1160,8 @@ struct Als {
         FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), st));
         unsigned colparts = 0;
         if (T > 0)
-            colparts = reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, pinv, U[j], renormalize ? colpart : nullptr, mt,
+            colparts = reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, (uint32_t)(R + (R & 1)), pinv, U[j],
+                                      renormalize ? colpart : nullptr, mt,
                                       mem, cub, st);
         else FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
         record(3);
@@ -1570,7 +1583,8 @@ int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* sa
         double* res = tmp.alloc<double>(result_rows * R);
         FTL_HIP_CHECK(hipMemcpyAsync(lhs, lhs_host, Jp * R * sizeof(double), hipMemcpyHostToDevice, st));
         FTL_HIP_CHECK(hipMemsetAsync(res, 0, result_rows * R * sizeof(double), st));
-        if (T > 0) reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, nullptr, res, nullptr, S.mt, S.mem, S.cub, st);
+        if (T > 0)
+            reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, (uint32_t)R, nullptr, res, nullptr, S.mt, S.mem, S.cub, st);
         FTL_HIP_CHECK(hipMemcpyAsync(result_host, res, result_rows * R * sizeof(double), hipMemcpyDeviceToHost, st));
         FTL_HIP_CHECK(hipStreamSynchronize(st));
         if (found) *found = T;

@@ -214,6 +214,8 @@ __global__ void k_dedup_heads(const uint64_t* __restrict__ skeys, const uint32_t
 
 // w' = w[first] * count, h' = h[first] * sqrt(w'), h'' = h' * sqrt(w')
 // (als.hpp:136-170: the same two roundings).
+// hdd rows have the even stride Rp = R + (R & 1) (a zero pad column for odd
+// R), so the MTTKRP reads each row in 16-byte column pairs.
 __global__ void k_dedup_rows(const uint32_t* __restrict__ ufirst, const uint32_t* __restrict__ ustart,
                              const uint32_t* __restrict__ counters, uint64_t J, uint32_t R,
                              const double* __restrict__ h, const double* __restrict__ w,
@@ -221,6 +223,7 @@ __global__ void k_dedup_rows(const uint32_t* __restrict__ ufirst, const uint32_t
 #pragma clang fp contract(off)
     const uint64_t Jp = counters[kCntUnique];
     const uint64_t n = Jp * R;
+    const uint64_t Rp = R + (R & 1);
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t u = e / R, c = e - u * R;
         const uint64_t f = ufirst[u];
@@ -228,7 +231,8 @@ __global__ void k_dedup_rows(const uint32_t* __restrict__ ufirst, const uint32_t
         const double sq = sqrt(wd);
         const double a = h[f * R + c] * sq;
         hs[e] = a;
-        hdd[e] = a * sq;
+        hdd[u * Rp + c] = a * sq;
+        if (Rp != R && c == R - 1) hdd[u * Rp + R] = 0.0;
     }
     (void)J;
 }
@@ -695,6 +699,67 @@ __global__ void __launch_bounds__(256) k_item_partial(const uint32_t* __restrict
         for (int q = 0; q < CPL; q++) {
             const uint32_t c = lane + q * G;
             if (c < R) partial[(uint64_t)w * R + c] = acc[q];
+        }
+    }
+}
+
+// k_item_partial for lhs rows of even stride Rp (16-byte aligned): lane l of
+// a G-lane group sums columns 2l and 2l + 1, reading them with one 16-byte
+// load per match -- half the load instructions (and addresses) of one
+// column per lane.  Each column's sum is the same sequential sum over the
+// piece's matches in order, so the partials equal k_item_partial's.
+template <int G>
+__global__ void __launch_bounds__(256) k_item_partial_pairs(const uint32_t* __restrict__ seg_start,
+                                                            const uint32_t* __restrict__ piece_off,
+                                                            const uint32_t* __restrict__ piece_seg,
+                                                            const uint32_t* __restrict__ counters,
+                                                            const uint64_t* __restrict__ spay,
+                                                            const double* __restrict__ valj,
+                                                            const double* __restrict__ lhs, uint32_t R, uint32_t Rp,
+                                                            double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    const uint32_t W = counters[kCntPieces];
+    const uint32_t lane = threadIdx.x % G;
+    const uint32_t c0 = 2 * lane;
+    const bool on = c0 < R;
+    const uint64_t groups = (uint64_t)gridDim.x * (256 / G);
+    for (uint64_t w = (uint64_t)blockIdx.x * (256 / G) + threadIdx.x / G; w < W; w += groups) {
+        const uint32_t l = piece_seg[w];  // segment of piece w
+        const uint64_t i0 = (uint64_t)seg_start[l] + (uint64_t)(w - piece_off[l]) * kSegPiece;
+        const uint64_t i1 = min<uint64_t>(i0 + kSegPiece, seg_start[l + 1]);
+        double a0 = 0.0, a1 = 0.0;
+        for (uint64_t cb = i0; cb < i1; cb += G) {
+            const int cnt = (int)min<uint64_t>(G, i1 - cb);
+            uint32_t my_u = 0;
+            double my_v = 0.0;
+            if ((int)lane < cnt) {
+                const uint64_t pay = spay[cb + lane];
+                my_u = (uint32_t)(pay >> 32);
+                my_v = valj[pay & 0xffffffffull];
+            }
+            // the lhs pairs of 8 matches are fetched before they are added (in
+            // match order, so the sums are the same)
+            for (int t0 = 0; t0 < cnt; t0 += 8) {
+                double2 x[8];
+                double vv[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int t = t0 + q < cnt ? t0 + q : cnt - 1;
+                    const uint64_t u = (uint32_t)__shfl(my_u, t, G);
+                    vv[q] = __shfl(my_v, t, G);
+                    x[q] = on ? *reinterpret_cast<const double2*>(lhs + u * Rp + c0) : double2{0.0, 0.0};
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    if (t0 + q < cnt) {
+                        a0 = a0 + x[q].x * vv[q];
+                        a1 = a1 + x[q].y * vv[q];
+                    }
+            }
+        }
+        if (on) {
+            partial[(uint64_t)w * R + c0] = a0;
+            if (c0 + 1 < R) partial[(uint64_t)w * R + c0 + 1] = a1;
         }
     }
 }
