@@ -938,36 +938,57 @@ __global__ void __launch_bounds__(256) k_seg_final(const uint32_t* __restrict__ 
     }
     const uint64_t groups = (uint64_t)gridDim.x * GPB;
     // groups run independently: a group stages its row in its own LDS slot,
-    // and a group is at most one wave, whose LDS accesses are ordered.  The
-    // piece range and row of a group's next segment are fetched while the
-    // current one is summed.
+    // and a group is at most one wave, whose LDS accesses are ordered.  A
+    // group's next segment has its first piece's values in flight while the
+    // current one is summed and multiplied, and the one after that its piece
+    // range and row.
     uint64_t d = (uint64_t)blockIdx.x * GPB + gi;
-    uint32_t nx0 = 0, nx1 = 0, nxrow = 0;
+    uint32_t nx0 = 0, nx1 = 0, nxrow = 0, n20 = 0, n21 = 0, n2row = 0;
+    double nxv[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; q++) nxv[q] = 0.0;
     if (d < D) {
         nx0 = piece_off[d];
         nx1 = d + 1 < D ? piece_off[d + 1] : W;
         nxrow = seg_row[d];
+#pragma unroll
+        for (int q = 0; q < CPL; q++)
+            if (lane + q * G < R) nxv[q] = partial[(uint64_t)nx0 * R + lane + q * G];
+    }
+    if (d + groups < D) {
+        const uint64_t d2 = d + groups;
+        n20 = piece_off[d2];
+        n21 = d2 + 1 < D ? piece_off[d2 + 1] : W;
+        n2row = seg_row[d2];
     }
     for (; d < D; d += groups) {
         double mt[CPL];
         const uint32_t p0 = nx0;
         uint32_t p1 = nx1;
         const uint64_t row = nxrow;
-        const uint64_t dn = d + groups;
-        if (dn < D) {
-            nx0 = piece_off[dn];
-            nx1 = dn + 1 < D ? piece_off[dn + 1] : W;
-            nxrow = seg_row[dn];
+#pragma unroll
+        for (int q = 0; q < CPL; q++) mt[q] = nxv[q];
+        if (d + groups < D) {
+            nx0 = n20;
+            nx1 = n21;
+            nxrow = n2row;
+#pragma unroll
+            for (int q = 0; q < CPL; q++)
+                if (lane + q * G < R) nxv[q] = partial[(uint64_t)nx0 * R + lane + q * G];
+        }
+        if (d + 2 * groups < D) {
+            const uint64_t d2 = d + 2 * groups;
+            n20 = piece_off[d2];
+            n21 = d2 + 1 < D ? piece_off[d2 + 1] : W;
+            n2row = seg_row[d2];
         }
         if (p1 - p0 > kHeavyPieces) p1 = p0 + 1;  // k_seg_heavy_sum left the row's sum in piece p0
 #pragma unroll
         for (int q = 0; q < CPL; q++) {
             const uint32_t c = lane + q * G;
-            double s = 0.0;
-            if (c < R) {
-                s = partial[(uint64_t)p0 * R + c];
+            double s = mt[q];  // piece p0 (0 past R)
+            if (c < R)
                 for (uint32_t p = p0 + 1; p < p1; p++) s = s + partial[(uint64_t)p * R + c];
-            }
             mt[q] = s;
         }
         if constexpr (!SOLVE) {
