@@ -489,7 +489,8 @@ void reduce_heavy(const uint32_t* piece_off, uint32_t* counters, uint32_t* list,
 // (k_item_partial_pairs, R <= 64).
 template <int G, int CPL>
 unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, const double* lhs, uint32_t R,
-                       uint32_t ldl, const double* pinv, double* out, double* colsq, hipStream_t st) {
+                       uint32_t ldl, const double* pinv, double* out, double* colsq, hipEvent_t join,
+                       hipStream_t st) {
     constexpr int GPB = 256 / G;
     const unsigned blocks = grid_for(T, GPB, 16384);
     if (ldl % 2 == 0 && R <= 64 && ((uintptr_t)lhs & 15) == 0) {
@@ -505,6 +506,7 @@ unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, c
     }
     FTL_LAUNCH_CHECK();
     reduce_heavy(m.piece_off, m.counters, m.heavy, std::min<uint64_t>(T, X.dims[j]), R, m.partial, st);
+    if (join) FTL_HIP_CHECK(hipStreamWaitEvent(st, join, 0));
     const unsigned fblocks = grid_for(T, GPB, kSegBlocksMax);
     if (pinv)
         hipLaunchKernelGGL((k_seg_final<G, CPL, true>), dim3(fblocks), dim3(256), 0, st, m.seg_row, m.piece_off,
@@ -519,9 +521,11 @@ unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, c
 // Given probed ranges of n samples and T >= 1 matches: list the matches,
 // sort them by output row (stable), form the row segments and their pieces,
 // reduce, and write each touched row of out (times pinv when given).
+// join (or null): an event the row finalise waits for before it writes out
+// (out's zero fill on another stream).
 unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
-                        uint32_t ldl, const double* pinv, double* out, double* colsq, Matches& m, Arena& ar,
-                        CubTemp& cub, hipStream_t st) {
+                        uint32_t ldl, const double* pinv, double* out, double* colsq, hipEvent_t join, Matches& m,
+                        Arena& ar, CubTemp& cub, hipStream_t st) {
     m.ensure_t(ar, T, R);
     // owner of every match (flags and segid are free until the row sort)
     FTL_HIP_CHECK(hipMemsetAsync(m.flags, 0, T * sizeof(uint32_t), st));
@@ -557,9 +561,9 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
                        m.piece_mark);
     FTL_LAUNCH_CHECK();
     inclusive_max(cub, m.piece_mark, m.piece_seg, wcap, st);
-    if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
-    if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
-    return launch_reduce<64, 2>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, st);
+    if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
+    if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
+    return launch_reduce<64, 2>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,6 +989,10 @@ struct Als {
     double *gram_had = nullptr, *exact_partial = nullptr;
     uint64_t exact_partial_cap = 0;
     uint32_t* full_rank = nullptr;  // counters with kCntUnique = R (pinv at full target rank)
+    // the zero fill of U_j in a sampled update runs on a second stream beside
+    // the draw, dedup and probe (none of which reads U_j)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_zero = nullptr;
     // diagnostics
     bool timing = false;
     std::array<hipEvent_t, 5> ev{};
@@ -993,9 +1001,13 @@ struct Als {
 
     ~Als() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         if (sampler) sampler_delete(sampler);
         for (auto e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_zero) (void)hipEventDestroy(ev_zero);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1010,6 +1022,9 @@ struct Als {
         FTL_HIP_CHECK(hipGetDevice(&device));
         // blocking stream: ordered after the caller's default-stream work
         FTL_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamDefault));
+        FTL_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        FTL_HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        FTL_HIP_CHECK(hipEventCreateWithFlags(&ev_zero, hipEventDisableTiming));
         pinv_lds_attrs();
         mt.init(mem);
         for (uint32_t k = 0; k < N; k++) {
@@ -1122,6 +1137,13 @@ struct Als {
         if (lk) lk->draw_device(j, samples, h, w, draws_dev, st);
         else sampler_draw(sampler, j, samples, h, w, draws_dev, st);
         record(1);
+        // U_j is zeroed on the side stream while steps 2-3 and the match
+        // expansion and sort run (latency-bound; the draw before them is
+        // bandwidth-bound and would be slowed)
+        FTL_HIP_CHECK(hipEventRecord(ev_fork, st));
+        FTL_HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+        FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), side));
+        FTL_HIP_CHECK(hipEventRecord(ev_zero, side));
         // 2. sort / unique with counts, scaled rows, sampled Gram pinv
         const KeyLayout& DL = dlayout[j];
         const unsigned gJ = grid_for(J, 256);
@@ -1156,14 +1178,17 @@ struct Als {
         last_unique = Jp;
         last_found = T;
         record(2);
-        // 4. U_j = MTTKRP(h'') pinv on the touched rows, zero elsewhere
-        FTL_HIP_CHECK(hipMemsetAsync(U[j], 0, I[j] * R * sizeof(double), st));
+        // 4. U_j = MTTKRP(h'') pinv on the touched rows, zero elsewhere (the
+        //    zero fill is joined in reduce_matches, before the row finalise)
         unsigned colparts = 0;
         if (T > 0)
             colparts = reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, (uint32_t)(R + (R & 1)), pinv, U[j],
-                                      renormalize ? colpart : nullptr, mt,
+                                      renormalize ? colpart : nullptr, ev_zero, mt,
                                       mem, cub, st);
-        else FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
+        else {
+            FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
+            FTL_HIP_CHECK(hipStreamWaitEvent(st, ev_zero, 0));
+        }
         record(3);
         // 5. renormalize (only touched rows are nonzero) and rebuild tree j
         if (renormalize) {
@@ -1584,7 +1609,8 @@ int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* sa
         FTL_HIP_CHECK(hipMemcpyAsync(lhs, lhs_host, Jp * R * sizeof(double), hipMemcpyHostToDevice, st));
         FTL_HIP_CHECK(hipMemsetAsync(res, 0, result_rows * R * sizeof(double), st));
         if (T > 0)
-            reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, (uint32_t)R, nullptr, res, nullptr, S.mt, S.mem, S.cub, st);
+            reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, (uint32_t)R, nullptr, res, nullptr, nullptr, S.mt, S.mem,
+                           S.cub, st);
         FTL_HIP_CHECK(hipMemcpyAsync(result_host, res, result_rows * R * sizeof(double), hipMemcpyDeviceToHost, st));
         FTL_HIP_CHECK(hipStreamSynchronize(st));
         if (found) *found = T;

@@ -196,6 +196,35 @@ def test_factor_sets_and_generated_requests_mixed(ftl):
     assert srv.stats()["bytes_in_use"] == 0
 
 
+def test_device_factors_are_read_before_the_call_returns(ftl):
+    """add_factors / update_factor with torch device tensors: the tensors are
+    overwritten and freed right after the calls, and later draws still equal
+    the one-shot sampler on the original factors."""
+    import torch
+    print("[gpu] device factors read before return", flush=True)
+    rng = np.random.default_rng(23)
+    dims, R = [20000, 3000, 7000], 16
+    U = [rng.standard_normal((d, R)) for d in dims]
+    dev = torch.device("cuda", 0)
+    srv = ftl.Server(8 << 30, 4)
+    T = [torch.from_numpy(u).to(dev) for u in U]
+    fs = srv.add_factors(T, 4096, "fp32")
+    for t in T:
+        t.fill_(float("nan"))
+    del T
+    junk = [torch.full((d, R), 7.0, dtype=torch.float64, device=dev) for d in dims]  # reuse the freed blocks
+    got = srv.wait(srv.submit_on(fs, 4096, None, 3))
+    np.testing.assert_array_equal(got, _one_shot(ftl, U, 4096, R, None, 3, "fp32"))
+    U1 = rng.standard_normal((dims[1], R))
+    t1 = torch.from_numpy(U1).to(dev)
+    srv.update_factor(fs, 1, t1)
+    t1.fill_(float("nan"))
+    del t1, junk
+    got = srv.wait(srv.submit_on(fs, 4096, None, 4))
+    np.testing.assert_array_equal(got, _one_shot(ftl, [U[0], U1, U[2]], 4096, R, None, 4, "fp32"))
+    srv.remove_factors(fs)
+
+
 def test_factor_set_errors(ftl):
     srv = ftl.Server(1 << 30, 2)
     U = [np.ones((100, 4)), np.ones((50, 4))]
