@@ -492,3 +492,37 @@ def test_compact_fp32_trees_equal_the_full_build(ftl, oracle, dims, R, levels):
     saved = sum(8 * (oracle.tree_shape(I, min(R, I))["node_count"] - _fp64_nodes(oracle, I, R, levels))
                 * (-(-(R * (R + 1) // 2) // 16) * 16) for I in dims if _compact_expected(oracle, I, R, levels))
     assert b.info()["device_bytes"] - a.info()["device_bytes"] >= saved - 4096
+
+
+def test_compact_fp32_trees_random_shapes(ftl, oracle):
+    """Seeded random shapes (ragged and power-of-two trees, R from 5 to 64,
+    one to five FP32 levels): draws of the compact FP32 trees equal those of
+    the full build bitwise, whether or not a mode is held compact."""
+    rng = np.random.default_rng(2027)
+    J = 2000
+    held = 0
+    for case in range(16):
+        R = int(rng.choice([5, 8, 13, 16, 24, 32, 40, 48, 57, 64]))
+        levels = int(rng.integers(1, 6))
+        dims = [int(R * 2 ** rng.uniform(7, 12.5)) + int(rng.integers(0, R)) for _ in range(int(rng.integers(2, 5)))]
+        print(f"[gpu] random compact case {case}: dims={dims} R={R} fp32_levels={levels}", flush=True)
+        U = _factors(dims, R, 5000 + case)
+        a = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+        a.set_option("fp32_levels", levels)
+        b = ftl.Sampler(U, J, R, "efficient", precision="fp32")
+        b.set_option("compact_fp32", 0)
+        b.set_option("fp32_levels", levels)
+        held += a.get_option("compact_modes")
+        assert a.get_option("compact_modes") == sum(1 for I in dims if _compact_expected(oracle, I, R, levels))
+        draws = rng.random(2 * len(dims) * J)
+        sa = np.zeros((len(dims), J), dtype=np.uint64)
+        sb = np.zeros((len(dims), J), dtype=np.uint64)
+        wa, wb = np.zeros(J), np.zeros(J)
+        ha, hb = np.zeros((J, R)), np.zeros((J, R))
+        a.KRPDrawSamples_materialize(len(dims) + 1, sa, ha, wa, draws)
+        b.KRPDrawSamples_materialize(len(dims) + 1, sb, hb, wb, draws)
+        np.testing.assert_array_equal(sa, sb)
+        np.testing.assert_array_equal(wa, wb)
+        a.close()
+        b.close()
+    assert held > 10  # most cases exercise the subtree build
