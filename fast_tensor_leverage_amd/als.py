@@ -63,6 +63,8 @@ class SparseTensor:
                                        None if dims_arr is None else _ptr(dims_arr), ctypes.byref(h)),
                    "SparseTensor")
         self._h = h
+        # held by the object: module globals may be gone when it is collected at exit
+        self._destroy = L.ftl_sparse_destroy
         info = np.zeros(2 + N, dtype=np.uint64)
         _lib.check(L.ftl_sparse_info(self._h, _ptr(info)))
         self.N = N
@@ -98,6 +100,7 @@ class SparseTensor:
                                                         None if dims_arr is None else _ptr(dims_arr),
                                                         ctypes.byref(h)), "SparseTensor")
         obj._h = h
+        obj._destroy = _lib.load().ftl_sparse_destroy
         info = np.zeros(2 + N, dtype=np.uint64)
         _lib.check(_lib.load().ftl_sparse_info(obj._h, _ptr(info)))
         obj.N, obj.nnz, obj.dims, obj.last_found = N, nnz, [int(d) for d in info[2:]], 0
@@ -109,7 +112,7 @@ class SparseTensor:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.load().ftl_sparse_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def get_normsq(self) -> float:
@@ -183,12 +186,13 @@ class _FactorHandle:
         _lib.check(_lib.load().ftl_als_create(X._h if X is not None else None, len(U_list), _ptr(I), R,
                                               _ptr_list(U_list), ctypes.byref(h)), "ALS")
         self._h = h
+        self._destroy = _lib.load().ftl_als_destroy
         _lib.check(_lib.load().ftl_als_set_sigma(self._h, _ptr(sigma)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.load().ftl_als_destroy(h)
+            self._destroy(h)
             self._h = None
 
 

@@ -30,6 +30,8 @@ class Server:
         _lib.check(_lib.load().ftl_server_create(int(hbm_budget_bytes), int(max_active), ctypes.byref(h)),
                    "Server")
         self._h = h
+        # held by the object: module globals may be gone when it is collected at exit
+        self._destroy = _lib.load().ftl_server_destroy
         self.max_active = int(max_active)
         self._shapes = {}
         self._sets = {}
@@ -37,7 +39,7 @@ class Server:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.load().ftl_server_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def submit(self, dims, R: int, J: int, j=None, seed: int = 0, precision: str = "fp64") -> int:

@@ -231,13 +231,15 @@ class OracleSampler:
         dims = np.array([U.shape[0] for U in self.U], dtype=np.uint64)
         self._ptrs = ptrs
         self._h = lib().ftlo_sampler_create(ptrs, _p(dims), self.N, self.J, self.R)
+        # held by the object: module globals may be gone when it is collected at exit
+        self._destroy = lib().ftlo_sampler_destroy
         self.h = np.ones((self.J, self.R))
         self.weights = np.zeros(self.J)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().ftlo_sampler_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def update_sampler(self, j: int, Uj=None):
@@ -304,11 +306,13 @@ class OracleSparse:
         self.nnz, self.N = self.idx.shape
         self.dims = [int(self.idx[:, k].max()) + 1 if self.nnz else 1 for k in range(self.N)]
         self._h = lib().ftlo_sparse_create(_p(self.idx), _p(self.values), self.nnz, self.N)
+        # held by the object: module globals may be gone when it is collected at exit
+        self._destroy = lib().ftlo_sparse_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().ftlo_sparse_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def normsq(self) -> float:
@@ -407,11 +411,13 @@ class OracleLKSampler:
         self.hybrid = bool(hybrid)
         self._I = np.array([u.shape[0] for u in self.U], dtype=np.uint64)
         self._h = lib().ftlo_lk_create2(_ptrs(self.U), _p(self._I), self.N, self.J, self.R, int(self.hybrid))
+        # held by the object: module globals may be gone when it is collected at exit
+        self._destroy = lib().ftlo_lk_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().ftlo_lk_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def update_sampler(self, j: int, Uj=None):
