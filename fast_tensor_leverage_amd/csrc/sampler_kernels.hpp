@@ -783,11 +783,15 @@ template <int RMAX, int SG>
 __device__ inline void make_coefs(const double* xs, const uint16_t* pt, int T, int g,
                                   double (&c)[DrawCfg<RMAX, SG>::EPL]) {
     using Cfg = DrawCfg<RMAX, SG>;
+    // the table entries are read at every call rather than hoisted: held in
+    // registers they would stay live through every descent
+    int gg = g;
+    asm volatile("" : "+v"(gg));
 #pragma unroll
     for (int i = 0; i < Cfg::NLD; i++) {
 #pragma unroll
         for (int v = 0; v < Cfg::VEC; v++) {
-            const int e = Cfg::VEC * (g + SG * i) + v;
+            const int e = Cfg::VEC * (gg + SG * i) + v;
             double val = 0.0;
             if (e < T) {
                 const uint32_t ab = pt[e];
@@ -958,9 +962,13 @@ __device__ inline uint32_t leaf_scan(const TU* __restrict__ U, uint64_t r0, uint
                     for (int w = 0; w < W; w++) buf[pp][q * W + w] = v[w];
                 }
             } else {
+                // formed per pass: hoisted, the eight clamped offsets would
+                // hold registers through every descent of the kernel
+                int cb = col0;
+                asm volatile("" : "+v"(cb));
 #pragma unroll
                 for (int w = 0; w < LC; w++) {
-                    const int b = col0 + w;
+                    const int b = cb + w;
                     buf[pp][w] = row[b < R ? b : R - 1];
                 }
             }
