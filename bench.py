@@ -142,6 +142,7 @@ def als_main(args):
         log(f"coordinate hash of every mode built in {t_hash:.3f} s")
     X = ftl.SparseTensor.__new__(ftl.SparseTensor)
     X._h, X.N, X.nnz, X.dims, X.last_found = h, N, nnz, list(dims), 0
+    X._destroy = L.ftl_sparse_destroy
     host_idx = host_vals = None
     if args.als_oracle:
         host_idx = idx.cpu().numpy().view(np.uint32).copy()

@@ -112,7 +112,8 @@ class SparseTensor:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            self._destroy(h)
+            destroy = getattr(self, "_destroy", None) or _lib.load().ftl_sparse_destroy
+            destroy(h)
             self._h = None
 
     def get_normsq(self) -> float:

@@ -1,7 +1,11 @@
 // als.hip -- host side of the device sparse tensor and the sampled CP-ALS
 // step: sorted lookups, the ALS update pipeline, renormalization and the
 // exact fit, and their C ABI (include/ftl_hip.h).
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/functional.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <array>
@@ -82,8 +86,8 @@ struct StreamDrain {
     }
 };
 
-// Growable temporary storage of the hipCUB primitives.
-struct CubTemp {
+// Growable temporary storage of the rocPRIM primitives.
+struct PrimTemp {
     void* p = nullptr;
     size_t bytes = 0;
     void* ensure(size_t b) {
@@ -95,41 +99,41 @@ struct CubTemp {
         }
         return p;
     }
-    ~CubTemp() {
+    ~PrimTemp() {
         if (p) (void)hipFree(p);
     }
 };
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
 template <typename K, typename V>
-void sort_pairs(CubTemp& t, const K* kin, K* kout, const V* vin, V* vout, uint64_t n, uint32_t bits,
+void sort_pairs(PrimTemp& t, const K* kin, K* kout, const V* vin, V* vout, uint64_t n, uint32_t bits,
                 hipStream_t st) {
     const int end_bit = (int)std::max<uint32_t>(1, std::min<uint32_t>(bits, 8 * sizeof(K)));
     size_t b = 0;
-    FTL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, kin, kout, vin, vout, n, 0, end_bit, st));
+    FTL_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, b, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
     void* p = t.ensure(b);
-    FTL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(p, b, kin, kout, vin, vout, n, 0, end_bit, st));
+    FTL_HIP_CHECK(rocprim::radix_sort_pairs(p, b, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
 }
 
-void inclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+void inclusive_sum(PrimTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
     size_t b = 0;
-    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b, in, out, n, st));
+    FTL_HIP_CHECK(rocprim::inclusive_scan(nullptr, b, in, out, (size_t)n, rocprim::plus<uint32_t>(), st));
     void* p = t.ensure(b);
-    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(p, b, in, out, n, st));
+    FTL_HIP_CHECK(rocprim::inclusive_scan(p, b, in, out, (size_t)n, rocprim::plus<uint32_t>(), st));
 }
 
-void inclusive_max(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+void inclusive_max(PrimTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
     size_t b = 0;
-    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b, in, out, hipcub::Max(), n, st));
+    FTL_HIP_CHECK(rocprim::inclusive_scan(nullptr, b, in, out, (size_t)n, rocprim::maximum<uint32_t>(), st));
     void* p = t.ensure(b);
-    FTL_HIP_CHECK(hipcub::DeviceScan::InclusiveScan(p, b, in, out, hipcub::Max(), n, st));
+    FTL_HIP_CHECK(rocprim::inclusive_scan(p, b, in, out, (size_t)n, rocprim::maximum<uint32_t>(), st));
 }
 
-void exclusive_sum(CubTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
+void exclusive_sum(PrimTemp& t, const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t st) {
     size_t b = 0;
-    FTL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
+    FTL_HIP_CHECK(rocprim::exclusive_scan(nullptr, b, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), st));
     void* p = t.ensure(b);
-    FTL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p, b, in, out, n, st));
+    FTL_HIP_CHECK(rocprim::exclusive_scan(p, b, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), st));
 }
 
 #define FTL_LAUNCH_CHECK() FTL_HIP_CHECK(hipGetLastError())
@@ -234,7 +238,7 @@ struct SparseDev {
     double* val[kMaxModes] = {};
     double normsq = 0.0;
     Arena mem;
-    CubTemp cub;
+    PrimTemp prim;
     Matches mt;
     uint32_t* tup = nullptr;
     uint64_t tup_cap = 0;
@@ -352,7 +356,7 @@ struct SparseDev {
             val[j] = mem.alloc<double>(nnz);
             hipLaunchKernelGGL(k_nz_keys, dim3(g), dim3(256), 0, stream, d_idx, nnz, layout[j], kin, pin);
             FTL_LAUNCH_CHECK();
-            sort_pairs(cub, kin, keys[j], pin, perm[j], nnz, layout[j].total_bits, stream);
+            sort_pairs(prim, kin, keys[j], pin, perm[j], nnz, layout[j].total_bits, stream);
             hipLaunchKernelGGL(k_nz_gather, dim3(g), dim3(256), 0, stream, d_idx, d_val, perm[j], nnz, N, j, row[j],
                                val[j]);
             FTL_LAUNCH_CHECK();
@@ -400,15 +404,15 @@ struct SparseDev {
         const unsigned g = grid_for(nnz, 256, 8192);
         hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, stream, iota, nnz);
         FTL_LAUNCH_CHECK();
-        sort_pairs(cub, row[j], srow, iota, o.pos, nnz, bits[j], stream);
+        sort_pairs(prim, row[j], srow, iota, o.pos, nnz, bits[j], stream);
         hipLaunchKernelGGL(k_head_flags<uint32_t>, dim3(g), dim3(256), 0, stream, srow, nnz, flags);
         FTL_LAUNCH_CHECK();
-        inclusive_sum(cub, flags, segid, nnz, stream);
+        inclusive_sum(prim, flags, segid, nnz, stream);
         hipLaunchKernelGGL(k_seg_heads, dim3(g), dim3(256), 0, stream, srow, segid, nnz, start, srows, o.counters);
         FTL_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_seg_pieces, dim3(g), dim3(256), 0, stream, start, o.counters, nnz, npieces);
         FTL_LAUNCH_CHECK();
-        exclusive_sum(cub, npieces, poff, nnz, stream);
+        exclusive_sum(prim, npieces, poff, nnz, stream);
         hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, stream, poff, npieces, nnz, o.counters,
                            (int)kCntPieces);
         FTL_LAUNCH_CHECK();
@@ -431,7 +435,7 @@ struct SparseDev {
         hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(o.D, 256, 8192)), dim3(256), 0, stream, o.piece_off, o.counters,
                            flags);
         FTL_LAUNCH_CHECK();
-        inclusive_max(cub, flags, o.piece_seg, o.W, stream);
+        inclusive_max(prim, flags, o.piece_seg, o.W, stream);
         hipLaunchKernelGGL(k_heavy_list, dim3(grid_for(o.D, 256, 2048)), dim3(256), 0, stream, o.piece_off, o.counters,
                            o.heavy);
         FTL_LAUNCH_CHECK();
@@ -461,12 +465,12 @@ struct SparseDev {
 // Probe n tuples (n from n_dev when given) on mode j's lookup: per-sample
 // ranges, their exclusive scan and the match total in counters[kCntMatches].
 void probe_and_count(SparseDev& X, uint32_t j, const uint32_t* tup, const uint32_t* n_dev, uint64_t n_host,
-                     uint64_t n_cap, Matches& m, Arena& ar, CubTemp& cub, hipStream_t st) {
+                     uint64_t n_cap, Matches& m, Arena& ar, PrimTemp& prim, hipStream_t st) {
     m.ensure_n(ar, n_cap);
     ProbeArgs pa = X.probe_args(j, tup, n_dev, n_host, n_cap, m);
     hipLaunchKernelGGL(k_probe, dim3(grid_for(n_cap, 256)), dim3(256), 0, st, pa);
     FTL_LAUNCH_CHECK();
-    exclusive_sum(cub, m.cnt, m.off, n_cap, st);
+    exclusive_sum(prim, m.cnt, m.off, n_cap, st);
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.off, m.cnt, n_cap, m.counters, (int)kCntMatches);
     FTL_LAUNCH_CHECK();
 }
@@ -525,20 +529,20 @@ unsigned launch_reduce(const SparseDev& X, uint32_t j, Matches& m, uint64_t T, c
 // (out's zero fill on another stream).
 unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const double* lhs, uint32_t R,
                         uint32_t ldl, const double* pinv, double* out, double* colsq, hipEvent_t join, Matches& m,
-                        Arena& ar, CubTemp& cub, hipStream_t st) {
+                        Arena& ar, PrimTemp& prim, hipStream_t st) {
     m.ensure_t(ar, T, R);
     // owner of every match (flags and segid are free until the row sort)
     FTL_HIP_CHECK(hipMemsetAsync(m.flags, 0, T * sizeof(uint32_t), st));
     hipLaunchKernelGGL(k_mark_owners, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, m.off, m.cnt, n, m.flags);
     FTL_LAUNCH_CHECK();
-    inclusive_max(cub, m.flags, m.segid, T, st);
+    inclusive_max(prim, m.flags, m.segid, T, st);
     hipLaunchKernelGGL(k_expand, dim3(grid_for(T, 256)), dim3(256), 0, st, m.segid, m.off, m.lo, T, X.row[j],
                        m.mrow, m.mpay);
     FTL_LAUNCH_CHECK();
-    sort_pairs(cub, m.mrow, m.srow, m.mpay, m.spay, T, X.bits[j], st);
+    sort_pairs(prim, m.mrow, m.srow, m.mpay, m.spay, T, X.bits[j], st);
     hipLaunchKernelGGL(k_head_flags<uint32_t>, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, T, m.flags);
     FTL_LAUNCH_CHECK();
-    inclusive_sum(cub, m.flags, m.segid, T, st);
+    inclusive_sum(prim, m.flags, m.segid, T, st);
     hipLaunchKernelGGL(k_seg_heads, dim3(grid_for(T, 256)), dim3(256), 0, st, m.srow, m.segid, T, m.seg_start,
                        m.seg_row, m.counters);
     FTL_LAUNCH_CHECK();
@@ -551,7 +555,7 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
     hipLaunchKernelGGL(k_seg_pieces, dim3(grid_for(dcap, 256)), dim3(256), 0, st, m.seg_start, m.counters, dcap,
                        m.npieces);
     FTL_LAUNCH_CHECK();
-    exclusive_sum(cub, m.npieces, m.piece_off, dcap, st);
+    exclusive_sum(prim, m.npieces, m.piece_off, dcap, st);
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, m.piece_off, m.npieces, dcap, m.counters,
                        (int)kCntPieces);
     FTL_LAUNCH_CHECK();
@@ -560,7 +564,7 @@ unsigned reduce_matches(SparseDev& X, uint32_t j, uint64_t n, uint64_t T, const 
     hipLaunchKernelGGL(k_mark_segs, dim3(grid_for(dcap, 256, 8192)), dim3(256), 0, st, m.piece_off, m.counters,
                        m.piece_mark);
     FTL_LAUNCH_CHECK();
-    inclusive_max(cub, m.piece_mark, m.piece_seg, wcap, st);
+    inclusive_max(prim, m.piece_mark, m.piece_seg, wcap, st);
     if (R <= 32) return launch_reduce<32, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
     if (R <= 64) return launch_reduce<64, 1>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
     return launch_reduce<64, 2>(X, j, m, T, lhs, R, ldl, pinv, out, colsq, join, st);
@@ -626,7 +630,7 @@ struct LKSampler {
     double *alpha = nullptr, *cuts = nullptr, *bmax = nullptr, *hpp = nullptr, *hpoff = nullptr, *pdet = nullptr;
     uint64_t *cand[kMaxModes] = {}, *ncand = nullptr, *hoff = nullptr, *ndet = nullptr;
     uint32_t *hcnt = nullptr, *hfail = nullptr;
-    CubTemp sel;
+    PrimTemp sel;
     // last draw: deterministic samples, their mass, candidates, 1 if the
     // enumeration was skipped, samples whose rejection attempts ran out
     double info[5] = {};
@@ -712,8 +716,8 @@ struct LKSampler {
             ndet = mem.alloc<uint64_t>(1);
             hfail = mem.alloc<uint32_t>(1);
             size_t bytes = 0;
-            FTL_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, hipcub::CountingInputIterator<uint64_t>(0),
-                                                   cand[0], ncand, (int)imax, LevAtLeast{lev[0], cuts}, stream));
+            FTL_HIP_CHECK(rocprim::select(nullptr, bytes, rocprim::counting_iterator<uint64_t>(0), cand[0], ncand,
+                                          (size_t)imax, LevAtLeast{lev[0], cuts}, stream));
             sel.ensure(bytes);
         }
         gparts = mem.alloc<double>((uint64_t)kGramBlocksMax * (R * (R + 1) / 2));
@@ -878,8 +882,8 @@ struct LKSampler {
         for (uint32_t k = 0; k < N; k++) {
             if ((int)k == j) continue;
             size_t bytes = sel.bytes;
-            FTL_HIP_CHECK(hipcub::DeviceSelect::If(sel.p, bytes, hipcub::CountingInputIterator<uint64_t>(0), cand[k],
-                                                   ncand + k, (int)I[k], LevAtLeast{lev[k], cuts + k}, st));
+            FTL_HIP_CHECK(rocprim::select(sel.p, bytes, rocprim::counting_iterator<uint64_t>(0), cand[k], ncand + k,
+                                          (size_t)I[k], LevAtLeast{lev[k], cuts + k}, st));
         }
         uint64_t nc[kMaxModes] = {};
         FTL_HIP_CHECK(hipMemcpyAsync(nc, ncand, N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -971,7 +975,7 @@ struct Als {
     double *sigma = nullptr, *norms = nullptr, *colpart = nullptr, *scal = nullptr, *resid_part = nullptr,
            *gram_parts = nullptr, *gram_red = nullptr;
     Arena mem;
-    CubTemp cub;
+    PrimTemp prim;
     Matches mt;
     // sampled update state (initialize_ds_als): the exact sampler, or the
     // Larsen-Kolda sampler
@@ -1149,10 +1153,10 @@ struct Als {
         const unsigned gJ = grid_for(J, 256);
         hipLaunchKernelGGL(k_sample_keys, dim3(gJ), dim3(256), 0, st, samples, J, DL, keys_a, vals_a);
         FTL_LAUNCH_CHECK();
-        sort_pairs(cub, keys_a, keys_b, vals_a, vals_b, J, DL.total_bits, st);
+        sort_pairs(prim, keys_a, keys_b, vals_a, vals_b, J, DL.total_bits, st);
         hipLaunchKernelGGL(k_head_flags<uint64_t>, dim3(gJ), dim3(256), 0, st, keys_b, J, flags);
         FTL_LAUNCH_CHECK();
-        inclusive_sum(cub, flags, uid, J, st);
+        inclusive_sum(prim, flags, uid, J, st);
         hipLaunchKernelGGL(k_dedup_heads, dim3(gJ), dim3(256), 0, st, keys_b, vals_b, uid, J, DL, utup, ufirst,
                            ustart, mt.counters);
         FTL_LAUNCH_CHECK();
@@ -1170,7 +1174,7 @@ struct Als {
         FTL_LAUNCH_CHECK();
         // 3. probe the unique tuples on mode j's lookup; the match count
         //    sizes the rest, so it is read back here (one wait per update)
-        probe_and_count(*X, j, utup, mt.counters + kCntUnique, 0, J, mt, mem, cub, st);
+        probe_and_count(*X, j, utup, mt.counters + kCntUnique, 0, J, mt, mem, prim, st);
         FTL_HIP_CHECK(hipMemcpyAsync(mt.host_counters, mt.counters, kCntSlots * sizeof(uint32_t),
                                      hipMemcpyDeviceToHost, st));
         FTL_HIP_CHECK(hipStreamSynchronize(st));
@@ -1184,7 +1188,7 @@ struct Als {
         if (T > 0)
             colparts = reduce_matches(*X, j, Jp, T, hdd, (uint32_t)R, (uint32_t)(R + (R & 1)), pinv, U[j],
                                       renormalize ? colpart : nullptr, ev_zero, mt,
-                                      mem, cub, st);
+                                      mem, prim, st);
         else {
             FTL_HIP_CHECK(hipMemsetAsync(mt.counters + kCntRows, 0, 2 * sizeof(uint32_t), st));
             FTL_HIP_CHECK(hipStreamWaitEvent(st, ev_zero, 0));
@@ -1575,7 +1579,7 @@ static uint64_t probe_host_samples(ftl::SparseDev& S, uint32_t j, uint64_t Jp, c
     FTL_HIP_CHECK(hipMemcpyAsync(stage, samples_host, Jp * S.N * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_tuples_u32, dim3(grid_for(Jp * S.N, 256)), dim3(256), 0, st, stage, Jp * S.N, tup);
     FTL_HIP_CHECK(hipGetLastError());
-    probe_and_count(S, j, tup, nullptr, Jp, Jp, S.mt, S.mem, S.cub, st);
+    probe_and_count(S, j, tup, nullptr, Jp, Jp, S.mt, S.mem, S.prim, st);
     hipLaunchKernelGGL(k_total_hi, dim3(1), dim3(1024), 0, st, S.mt.cnt, Jp, S.mt.counters, (int)kCntMatchesHi);
     FTL_HIP_CHECK(hipGetLastError());
     FTL_HIP_CHECK(hipMemcpyAsync(S.mt.host_counters, S.mt.counters, kCntSlots * sizeof(uint32_t),
@@ -1610,7 +1614,7 @@ int ftl_sparse_mttkrp(ftl_sparse* X, uint32_t j, uint64_t Jp, const uint64_t* sa
         FTL_HIP_CHECK(hipMemsetAsync(res, 0, result_rows * R * sizeof(double), st));
         if (T > 0)
             reduce_matches(S, j, Jp, T, lhs, (uint32_t)R, (uint32_t)R, nullptr, res, nullptr, nullptr, S.mt, S.mem,
-                           S.cub, st);
+                           S.prim, st);
         FTL_HIP_CHECK(hipMemcpyAsync(result_host, res, result_rows * R * sizeof(double), hipMemcpyDeviceToHost, st));
         FTL_HIP_CHECK(hipStreamSynchronize(st));
         if (found) *found = T;
