@@ -614,10 +614,34 @@ def lk_main(args):
     print(json.dumps(line), flush=True)
 
 
+def device_copy_bandwidth(dev, gib=4, reps=5):
+    """Bytes read plus written per second by a device-to-device copy of a
+    `gib` GiB tensor (torch's copy kernel, HIP events on its stream): the
+    bandwidth a streaming kernel reaches on this device, a second denominator
+    beside the 8 TB/s peak."""
+    import torch
+    n = gib * 2 ** 30 // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * n * 8 / sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--trials", type=int, default=7, help="single synchronised draws after the timed run")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=J_SAMPLES)
@@ -704,8 +728,16 @@ def main():
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         s.enable_timing(False)
+        # single calls, each synchronised on both sides: median and best
+        trial_s = []
+        for _ in range(args.trials):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            s.draw_device(j, d_samples, d_h, d_w, d_draws)
+            torch.cuda.synchronize()
+            trial_s.append(time.perf_counter() - t1)
         results[prec] = {"elapsed": elapsed, "timing": timing, "build_s": t_build, "info": info,
-                         "group_lanes": s.get_option("group_lanes") or 32}
+                         "group_lanes": s.get_option("group_lanes") or 32, "trial_s": trial_s}
         samples_out[prec] = d_samples.cpu().numpy().view(np.uint64).copy()
         log(f"{prec}: {args.steps * J / elapsed:,.0f} samples/s; kernels {timing}; build {t_build:.2f} s")
         del s, d_samples, d_h, d_w
@@ -736,6 +768,8 @@ def main():
         traffic = tj["fp32"]["hbm_bytes_per_draw"]
         traffic64 = tj["fp64"]["hbm_bytes_per_draw"]
     t64 = f64["timing"]["draw_ms"] * 1e-3
+    copy_bw = device_copy_bandwidth(dev)
+    log(f"device-to-device copy: {copy_bw / 1e9:,.0f} GB/s (read + write)")
 
     serving = None
     if not args.skip_serve:
@@ -775,7 +809,13 @@ def main():
                      "algorithmic_bytes_per_launch": bytes32 / N,
                      "algorithmic_bytes_per_sample_mode": bytes32 / (N * J),
                      "launches_per_draw": N,
-                     "stored_fp64_top_levels": fp64_levels},
+                     "stored_fp64_top_levels": fp64_levels,
+                     "copy_GBps": copy_bw / 1e9,
+                     "frac_of_copy": achieved / copy_bw},
+        "trials": {p: {"calls": len(results[p]["trial_s"]),
+                       "median_samples_per_s": J / float(np.median(results[p]["trial_s"])),
+                       "best_samples_per_s": J / float(np.min(results[p]["trial_s"])),
+                       "trial_ms": [1e3 * t for t in results[p]["trial_s"]]} for p in ("fp32", "fp64")},
         "cpu_baseline": cpu,
         "fp64_gpu_samples_per_s": v64,
         "fp32_over_fp64": value / v64,

@@ -160,11 +160,12 @@ inline size_t subtree_lds(uint64_t R, uint32_t depth, uint32_t d64, uint32_t wav
 constexpr size_t kSubtreeLds = 150u * 1024u;
 
 // LDS of a k_subtree_pairs block for subtrees L levels deep: a node per
-// wave (2^(L-1) waves) in the accumulator layout, and the (row, column) of
-// each packed entry.
+// wave (2^(L-1) waves) in the accumulator layout and the (row, column) of
+// each packed entry, or at R = 49-64 the packed triangle alone.
 inline size_t pairs_lds(uint64_t R, uint32_t L) {
-    const uint64_t nb = (R + 15) / 16, nt = nb * (nb + 1) / 2;
-    return (size_t)(1ull << (L - 1)) * nt * 256 * sizeof(double) + tri_size(R) * sizeof(uint16_t);
+    const uint64_t nb = (R + 15) / 16;
+    if (nb == 4) return (size_t)(1ull << (L - 1)) * pairs_slot_doubles<4>(R) * sizeof(double);
+    return (size_t)(1ull << (L - 1)) * (nb * (nb + 1) / 2 * 256) * sizeof(double) + tri_size(R) * sizeof(uint16_t);
 }
 
 // Whether an FP32 tree is held compact: FP32 levels below d64 exist, the

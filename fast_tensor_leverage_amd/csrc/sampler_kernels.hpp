@@ -327,18 +327,134 @@ __global__ void __launch_bounds__(64 * WPB) k_subtree_build(BuildArgs a, uint64_
 // wave's slot, from which the wave writes it in packed order (coalesced),
 // FP32 -- or FP64 for c.  Dynamic LDS: a node per wave in the accumulator
 // layout (NT x 4 x 64 doubles), then the (row, column) of each packed entry.
+//
+// At NB = 4 (R = 49-64) a slot in that layout is 20 KB, and eight waves
+// (four FP32 levels) would not fit the LDS: there the slot holds the packed
+// upper triangle instead (T entries, 16.6 KB at R = 64).  A lane then finds
+// the packed index of each tile entry it holds (row 16p + 4v + lane / 16,
+// column 16q + lane % 16) and moves only the entries on or above the
+// diagonal and inside R; the others (the lower half of diagonal tiles, the
+// zero padding past R) are never written out.
 template <int NB>
-__device__ inline void tiles_to_slot(const lg_f64x4 (&acc)[NB * (NB + 1) / 2], double* slot, uint32_t lane) {
+constexpr bool pairs_packed_slot() {
+    return NB == 4;
+}
+
+// Doubles of one wave's slot (packed: T rounded up to even, so every slot
+// starts 16-byte aligned).  Packed slots are written out entry by entry and
+// need no (row, column) table; at R = 49 eight of them take 78.5 KB, so two
+// blocks fit a CU's LDS.
+template <int NB>
+__host__ __device__ inline uint64_t pairs_slot_doubles(uint64_t R) {
+    return pairs_packed_slot<NB>() ? (tri_size(R) + 1) / 2 * 2 : (uint64_t)(NB * (NB + 1) / 2) * 256;
+}
+
+// Packed index of the entry of tile t (the t-th of the upper tile triangle)
+// that `lane` holds in accumulator v, or -1 when it is below the diagonal
+// or past R.
+template <int NB>
+__device__ inline int packed_tile_index(int t, int v, uint32_t lane, uint32_t R) {
+    int p = 0, rem = t;
+    while (rem >= NB - p) {
+        rem -= NB - p;
+        p++;
+    }
+    const int q = p + rem;
+    const int r = 16 * p + 4 * v + (int)(lane >> 4), c = 16 * q + (int)(lane & 15);
+    if (r > c || c >= (int)R) return -1;
+    return r * (int)R - r * (r - 1) / 2 + (c - r);
+}
+
+template <int NB>
+__device__ inline void tiles_to_slot(const lg_f64x4 (&acc)[NB * (NB + 1) / 2], double* slot, uint32_t lane,
+                                     uint32_t R) {
+    if constexpr (pairs_packed_slot<NB>()) {
+        // the lane index is made opaque so that the 40 packed indices are
+        // formed at each use instead of being held in registers
+        uint32_t ln = lane;
+        asm volatile("" : "+v"(ln));
 #pragma unroll
-    for (int t = 0; t < NB * (NB + 1) / 2; t++)
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) slot[(t * 4 + v) * 64 + lane] = acc[t][v];
+            for (int v = 0; v < 4; v++) {
+                const int e = packed_tile_index<NB>(t, v, ln, R);
+                if (e >= 0) slot[e] = acc[t][v];
+            }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) slot[(t * 4 + v) * 64 + lane] = acc[t][v];
+    }
+}
+
+// The slot holds the left sibling and acc the right one: afterwards acc holds
+// their sum (left + right) and the slot the right sibling, entry by entry
+// (each entry a lane holds is its own in either layout, so no other lane
+// reads it in between).
+template <int NB>
+__device__ inline void swap_add_slot(lg_f64x4 (&acc)[NB * (NB + 1) / 2], double* slot, uint32_t lane, uint32_t R) {
+    if constexpr (pairs_packed_slot<NB>()) {
+        uint32_t ln = lane;
+        asm volatile("" : "+v"(ln));
+#pragma unroll
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int e = packed_tile_index<NB>(t, v, ln, R);
+                if (e >= 0) {
+                    const double right = acc[t][v];
+                    acc[t][v] = slot[e] + right;
+                    slot[e] = right;
+                }
+            }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double& d = slot[(t * 4 + v) * 64 + lane];
+                const double right = acc[t][v];
+                acc[t][v] = d + right;
+                d = right;
+            }
+    }
+}
+
+// acc[t][v] = s[t][v] + the slot's entry (left + right, in that order: `slot_left`
+// says which side the slot holds); entries the slot does not hold keep acc.
+template <int NB>
+__device__ inline void add_slot(lg_f64x4 (&acc)[NB * (NB + 1) / 2], const double* slot, bool slot_left,
+                                uint32_t lane, uint32_t R) {
+    if constexpr (pairs_packed_slot<NB>()) {
+        uint32_t ln = lane;
+        asm volatile("" : "+v"(ln));
+#pragma unroll
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int e = packed_tile_index<NB>(t, v, ln, R);
+                if (e >= 0) acc[t][v] = slot_left ? slot[e] + acc[t][v] : acc[t][v] + slot[e];
+            }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NB * (NB + 1) / 2; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double d = slot[(t * 4 + v) * 64 + lane];
+                acc[t][v] = slot_left ? d + acc[t][v] : acc[t][v] + d;
+            }
+    }
 }
 
 // Entries [0, ts) of a node from its wave's slot (zero from T on), FP32 or FP64.
 template <int NB, typename OutT>
 __device__ inline void slot_write(const double* slot, const uint16_t* pairs, OutT* out, uint32_t T, uint32_t ts,
                                   uint32_t lane) {
+    if constexpr (pairs_packed_slot<NB>()) {
+        for (uint32_t e = lane; e < ts; e += 64) out[e] = (OutT)(e < T ? slot[e] : 0.0);
+        return;
+    }
     for (uint32_t e = lane; e < ts; e += 64) {
         double v = 0.0;
         if (e < T) {
@@ -359,14 +475,17 @@ __global__ void __launch_bounds__(NB <= 2 ? 1024 : 512)
     const uint32_t R = a.R, T = (uint32_t)tri_size(R);
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t waves = blockDim.x >> 6;
-    uint16_t* pairs_rc = reinterpret_cast<uint16_t*>(xch + (uint64_t)waves * NT * 256);
-    for (uint32_t r = threadIdx.x; r < R; r += blockDim.x) {
-        const uint32_t base = r * R - r * (r - 1) / 2;
-        for (uint32_t c = r; c < R; c++) pairs_rc[base + c - r] = (uint16_t)(r | (c << 8));
+    const uint64_t slot_n = pairs_slot_doubles<NB>(R);
+    uint16_t* pairs_rc = reinterpret_cast<uint16_t*>(xch + (uint64_t)waves * slot_n);
+    if constexpr (!pairs_packed_slot<NB>()) {
+        for (uint32_t r = threadIdx.x; r < R; r += blockDim.x) {
+            const uint32_t base = r * R - r * (r - 1) / 2;
+            for (uint32_t c = r; c < R; c++) pairs_rc[base + c - r] = (uint16_t)(r | (c << 8));
+        }
     }
     __syncthreads();
     const uint64_t first_leaf = a.first_leaf, nodes = a.sh.node_count;
-    double* mine = xch + (uint64_t)w * NT * 256;
+    double* mine = xch + (uint64_t)w * slot_n;
     auto emit = [&](uint64_t node, bool f64) {
         wave_sync_lds();
         if (f64) slot_write<NB>(mine, pairs_rc, a.tree + node * (uint64_t)a.ts64, T, a.ts64, lane);
@@ -380,7 +499,7 @@ __global__ void __launch_bounds__(NB <= 2 ? 1024 : 512)
             if (w == 0) {
                 lg_f64x4 acc[NT];
                 leaf_mfma_tiles<NB>(a, c, lane, acc);
-                tiles_to_slot<NB>(acc, mine, lane);
+                tiles_to_slot<NB>(acc, mine, lane, R);
                 emit(c, true);
             }
             continue;
@@ -393,37 +512,25 @@ __global__ void __launch_bounds__(NB <= 2 ? 1024 : 512)
         if (w < npairs) {
             const uint64_t la = ((c + 1) << Lc) - 1 + 2 * (uint64_t)w;
             leaf_mfma_tiles<NB>(a, la, lane, A);
-            tiles_to_slot<NB>(A, mine, lane);
+            tiles_to_slot<NB>(A, mine, lane, R);
             emit(la, false);
             leaf_mfma_tiles<NB>(a, la + 1, lane, A);
-            // the parent: each lane adds the left leaf's entries it holds itself
-            lg_f64x4 S[NT];
-#pragma unroll
-            for (int t = 0; t < NT; t++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) S[t][v] = mine[(t * 4 + v) * 64 + lane] + A[t][v];
-            wave_sync_lds();
-            tiles_to_slot<NB>(A, mine, lane);
+            // the parent in A, the right leaf in the slot: each lane swaps
+            // the entries it holds itself
+            swap_add_slot<NB>(A, mine, lane, R);
             emit(la + 1, false);
-#pragma unroll
-            for (int t = 0; t < NT; t++) A[t] = S[t];
-            tiles_to_slot<NB>(A, mine, lane);
+            tiles_to_slot<NB>(A, mine, lane, R);
             nd = (la - 1) >> 1;
             emit(nd, nd == c);
         }
         for (uint32_t k = 1; k < Lc; k++) {
             const uint32_t half = 1u << (k - 1);
             __syncthreads();  // the right children's slots hold their nodes
-            if (w < npairs && (w & (2 * half - 1)) == 0) {  // a left child: add its sibling
-                const double* d = xch + (uint64_t)(w + half) * NT * 256;
-#pragma unroll
-                for (int t = 0; t < NT; t++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++) A[t][v] = A[t][v] + d[(t * 4 + v) * 64 + lane];
-            }
+            if (w < npairs && (w & (2 * half - 1)) == 0)  // a left child: add its sibling
+                add_slot<NB>(A, xch + (uint64_t)(w + half) * slot_n, false, lane, R);
             __syncthreads();  // the sibling slots are read before the left children rewrite theirs
             if (w < npairs && (w & (2 * half - 1)) == 0) {
-                tiles_to_slot<NB>(A, mine, lane);
+                tiles_to_slot<NB>(A, mine, lane, R);
                 nd = (nd - 1) >> 1;
                 emit(nd, nd == c);
             }

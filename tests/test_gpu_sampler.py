@@ -41,8 +41,8 @@ def _compact_expected(oracle, I, R, fp32_levels):
     """Whether an FP32 sampler holds mode I's FP32 levels in FP32 only: it has
     FP32 levels, R <= 64, at least 64 nodes at the lowest FP64 level, the
     subtrees below it at most 5 levels deep (4 above R = 32), and a block's
-    LDS (a 16 x 16-tile node per leaf pair, or four leaves and one node per
-    depth) within 150 KB."""
+    LDS (a node per leaf pair -- 16 x 16 tiles, or the packed triangle at
+    R = 49-64 -- or four leaves and one node per depth) within 150 KB."""
     sh = oracle.tree_shape(I, min(R, I))
     depth = sh["lfill_level"] + (1 if sh["node_count"] > sh["nodes_upto_lfill"] else 0)
     d64 = max(1, depth + 1 - fp32_levels)
@@ -52,8 +52,9 @@ def _compact_expected(oracle, I, R, fp32_levels):
     nb = -(-R // 16)
     T = R * (R + 1) // 2
     ts64 = -(-T // 16) * 16
-    return (L <= (5 if R <= 32 else 4) and 2 ** (L - 1) * nb * (nb + 1) // 2 * 2048 + 2 * T <= 150 * 1024
-            and (4 + L) * ts64 * 8 <= 150 * 1024)
+    # a slot per wave: 16 x 16 tiles and the (row, column) table, or the packed triangle at R = 49-64
+    pairs = 2 ** (L - 1) * -(-T // 2) * 2 * 8 if nb == 4 else 2 ** (L - 1) * nb * (nb + 1) // 2 * 2048 + 2 * T
+    return (L <= (5 if R <= 32 else 4) and pairs <= 150 * 1024 and (4 + L) * ts64 * 8 <= 150 * 1024)
 
 
 def _fp64_nodes(oracle, I, R, fp32_levels):
@@ -442,7 +443,9 @@ def test_fp32_sampler_matrix_core_leaf_grams(ftl, oracle, R):
     ([2 ** 15 + 3, 900, 64], 25, 7),          # seven FP32 levels: too deep for one block, full build
     ([60000, 15000, 130], 48, 4),
     ([90000, 70000, 65], 64, 3),              # and a factor of fewer rows than two leaves' worth
-    ([9000, 700, 65], 64, 4),                 # R = 64 four levels deep: over the LDS budget, full build
+    ([9000, 700, 65], 64, 4),                 # R = 64, trees too small to hold compact: full build
+    ([2 ** 16 + 5, 40000, 64], 64, 4),        # R = 64 four levels deep: packed-triangle slots
+    ([70001, 2 ** 16, 300], 49, 4),           # R = 49: packed slots, partial last tile
 ])
 def test_compact_fp32_trees_equal_the_full_build(ftl, oracle, dims, R, levels):
     """FP32 samplers hold their FP32 levels only in FP32 (the subtree build):
