@@ -139,12 +139,13 @@ def tune_bucket(ftl, torch, dev, precision, rb, db, jb, steps, mismatch_bar, rou
         from .synth import ill_conditioned_factors
         key = (R, I)
         if key not in _LEVEL_CACHE:
-            levels = [lv for lv in (7, 6, 5, 4, 3, 2) if lv <= depth + 1]
+            levels = [lv for lv in (7, 6, 5, 4, 3, 2, 1) if lv <= depth + 1]
             r_g = _mismatch(ftl, torch, U, R, 50_000, levels, dev, 11, rounds)
             Ui, _ = ill_conditioned_factors(torch, N_MODES, I, R, dev, seed=31 + R)
             r_i = _mismatch(ftl, torch, Ui, R, 50_000, levels, dev, 12, rounds)
             del Ui
             ok = [lv for lv in levels if max(r_g[lv], r_i[lv]) <= mismatch_bar]
+            # none under the bar: the fewest FP32 levels an FP32 sampler has
             _LEVEL_CACHE[key] = (max(ok) if ok else 1, r_g, r_i)
         lv, r_g, r_i = _LEVEL_CACHE[key]
         opts["fp32_levels"] = lv
