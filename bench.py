@@ -213,6 +213,7 @@ def als_main(args):
         "wall_s_updates_and_fits": t_total,
         "sparse_construction_s": t_build,
         "hash_construction_s": t_hash,
+        "eigensolves_capped": ftl.eigensolver_capped(),
         "generation_s": t_gen,
     }
     if args.exact_sweeps:
