@@ -993,10 +993,14 @@ struct Als {
     double *gram_had = nullptr, *exact_partial = nullptr;
     uint64_t exact_partial_cap = 0;
     uint32_t* full_rank = nullptr;  // counters with kCntUnique = R (pinv at full target rank)
-    // the zero fill of U_j in a sampled update runs on a second stream beside
-    // the draw, dedup and probe (none of which reads U_j)
+    // the zero fill of U_j in a sampled update runs on a second stream after
+    // the draw (bandwidth-bound, it would be slowed), beside the dedup, probe
+    // and match sort (none of which reads U_j)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_zero = nullptr;
+    // mode excluded by the last sampled update (-1: none since the sampler
+    // was set up)
+    int last_sampled_j = -1;
     // diagnostics
     bool timing = false;
     std::array<hipEvent_t, 5> ev{};
@@ -1124,16 +1128,22 @@ struct Als {
     // zero), h (J x R) and weights (J), as the sampler wrote them.
     void last_draw(uint64_t* samples_host, double* h_host, double* w_host) {
         FTL_REQUIRE(sampler || lk, "initialize_ds_als has not been called");
+        FTL_REQUIRE(last_sampled_j >= 0, "no sampled update has run");
         FTL_HIP_CHECK(hipMemcpyAsync(samples_host, samples, N * J * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
         if (h_host) FTL_HIP_CHECK(hipMemcpyAsync(h_host, h, J * R * sizeof(double), hipMemcpyDeviceToHost, stream));
         if (w_host) FTL_HIP_CHECK(hipMemcpyAsync(w_host, w, J * sizeof(double), hipMemcpyDeviceToHost, stream));
         FTL_HIP_CHECK(hipStreamSynchronize(stream));
+        // the draw writes the sampled modes' rows only: the excluded mode's
+        // row of the device buffer still holds an earlier update's samples
+        std::fill(samples_host + (uint64_t)last_sampled_j * J, samples_host + (uint64_t)(last_sampled_j + 1) * J,
+                  (uint64_t)0);
     }
 
     // execute_ds_als_update(j, renormalize, update_sampler) (als.hpp:57-220)
     void ds_update(uint32_t j, bool renormalize, bool update_sampler, const double* draws_dev) {
         FTL_REQUIRE(sampler || lk, "initialize_ds_als has not been called");
         FTL_REQUIRE(j < N, "mode index out of range");
+        last_sampled_j = (int)j;
         have_events = timing;
         hipStream_t st = stream;
         record(0);

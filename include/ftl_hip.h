@@ -370,7 +370,8 @@ int ftl_als_enable_timing(ftl_als* a, int enable);
 
 /* The draw of the last sampled update (step 1 of execute_ds_als_update,
  * als.hpp:70-74): samples (N x J uint64, the excluded mode's row 0), h
- * (J x R, or NULL) and weights (J, or NULL), copied to the host. */
+ * (J x R, or NULL) and weights (J, or NULL), copied to the host.
+ * FTL_EINVAL before the first sampled update. */
 int ftl_als_last_draw(ftl_als* a, uint64_t* samples_host, double* h_host, double* w_host);
 
 /* Stream of the handle (hipStream_t). */

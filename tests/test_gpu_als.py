@@ -343,3 +343,41 @@ def test_hash_lookup_als_updates_equal_sort(ftl):
     assert out["sort"][1] == out["hash"][1]
     for a, b in zip(out["sort"][0], out["hash"][0]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_last_draw_is_the_update_draw(ftl, oracle):
+    """ALS.last_draw() returns the draw the last sampled update consumed: the
+    samples, h and weights of a separate FP64 sampler on the same factors and
+    uniforms, bitwise; and the oracle's update on those samples reproduces the
+    GPU update (the comparison the C5 benchmark makes at full size)."""
+    rng = np.random.default_rng(5151)
+    dims, R, J = (60, 50, 40), 8, 800
+    idx, vals = random_sparse(dims, 5000, rng, unique=False)
+    U0 = [rng.standard_normal((d, R)) for d in dims]
+    lr = ftl.LowRankTensor(R, [u.copy() for u in U0])
+    als = ftl.ALS(lr, ftl.SparseTensor(idx, vals))
+    als.initialize_ds_als(J, "efficient")
+    with pytest.raises(Exception, match="no sampled update"):
+        als.last_draw()
+    for j in (1, 0, 2):
+        before = [u.copy() for u in lr.U]
+        sigma_before = lr.sigma.copy()
+        draws = rng.random(2 * len(dims) * J)
+        als.execute_ds_als_update(j, True, True, random_draws=draws)
+        samples, h, w = als.last_draw()
+        ref = ftl.Sampler([u.copy() for u in before], J, R, "efficient", precision="fp64")
+        s2 = np.zeros((len(dims), J), dtype=np.uint64)
+        h2 = np.zeros((J, R))
+        w2 = np.zeros(J)
+        ref.KRPDrawSamples_materialize(j, s2, h2, w2, draws)
+        np.testing.assert_array_equal(samples, s2)
+        np.testing.assert_array_equal(h, h2)
+        np.testing.assert_array_equal(w, w2)
+        assert np.all(samples[j] == 0)
+        ref.close()
+        o = oracle.OracleALS(oracle.OracleSparse(idx, vals), [u.copy() for u in before], J, sigma=sigma_before)
+        d = o.update_with_samples(j, samples)
+        assert d["unique_samples"] == als.last_stats()["unique_samples"]
+        scale = np.abs(o.U[j]).max()
+        np.testing.assert_allclose(lr.U[j], o.U[j], rtol=1e-9, atol=1e-11 * scale)
+        np.testing.assert_allclose(lr.sigma, o.sigma, rtol=1e-10)
