@@ -391,6 +391,91 @@ def serve_main(args):
 
 
 # ---------------------------------------------------------------------------
+# the reference's runtime sweeps (--workload sweep)
+# ---------------------------------------------------------------------------
+# Published CPU means (seconds; 128 OpenMP threads of a Perlmutter node) of
+# benchmarks/runtime_benchmark.py, from submission_data/oct_30/runtime_bench.json
+# (I_trace: R=32, N=3; R_trace and N_trace: I=2^22): (I, R, N) -> (sampling, construction)
+REF_SWEEP = {
+    (2 ** 6, 32, 3): (0.0412, 0.0010), (2 ** 7, 32, 3): (0.0336, 0.0008), (2 ** 8, 32, 3): (0.0345, 0.0010),
+    (2 ** 9, 32, 3): (0.0418, 0.0014), (2 ** 10, 32, 3): (0.0437, 0.0014), (2 ** 11, 32, 3): (0.0451, 0.0016),
+    (2 ** 12, 32, 3): (0.0491, 0.0023), (2 ** 13, 32, 3): (0.0749, 0.0053), (2 ** 14, 32, 3): (0.0810, 0.0058),
+    (2 ** 15, 32, 3): (0.0686, 0.0074), (2 ** 16, 32, 3): (0.0705, 0.0097), (2 ** 17, 32, 3): (0.0742, 0.0141),
+    (2 ** 18, 32, 3): (0.1408, 0.0296), (2 ** 19, 32, 3): (0.1563, 0.0428), (2 ** 20, 32, 3): (0.1762, 0.0639),
+    (2 ** 21, 32, 3): (0.1771, 0.1244), (2 ** 22, 32, 3): (0.1885, 0.2438), (2 ** 23, 32, 3): (0.2189, 0.4806),
+    (2 ** 24, 32, 3): (0.2635, 0.9738), (2 ** 25, 32, 3): (0.3109, 2.0254),
+    (2 ** 22, 16, 3): (0.0835, 0.1505), (2 ** 22, 64, 3): (0.6744, 0.4978), (2 ** 22, 128, 3): (2.6808, 1.0201),
+    (2 ** 22, 32, 2): (0.1285, 0.1770), (2 ** 22, 32, 4): (0.2421, 0.3313), (2 ** 22, 32, 5): (0.3044, 0.4160),
+    (2 ** 22, 32, 6): (0.3804, 0.4933), (2 ** 22, 32, 7): (0.4460, 0.5732), (2 ** 22, 32, 8): (0.5099, 0.6523),
+    (2 ** 22, 32, 9): (0.5724, 0.7361),
+}
+
+
+def sweep_main(args):
+    """The configurations of the reference's runtime benchmark
+    (benchmarks/runtime_benchmark.py:33-118: I from 2^6 to 2^25 at R=32, N=3;
+    R from 16 to 128 and N from 2 to 9 at I=2^22; J=50,000; every mode sampled,
+    j = N+1), each with `trials` constructions and draws as there.  Timed like
+    the reference: construction = the sampler built from factors already in
+    memory (here HBM, N(0,1) from torch's generator), sampling = one
+    KRPDrawSamples with its built-in uniforms into a host buffer, synchronised.
+    Reported per configuration: the median over the trials for the FP64 and FP32
+    samplers, the reference's published CPU mean and the ratio."""
+    import torch
+    sys.path.insert(0, HERE)
+    import fast_tensor_leverage_amd as ftl
+    dev = torch.device("cuda", 0)
+    J = J_SAMPLES
+    rows = []
+    t_all = time.perf_counter()
+    for (I, R, N), (ref_s, ref_c) in REF_SWEEP.items():
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(I + 7 * R + 131 * N)
+        U = [torch.randn((I, R), generator=gen, device=dev, dtype=torch.float64) for _ in range(N)]
+        row = {"I": I, "R": R, "N": N, "ref_sampling_s": ref_s, "ref_construction_s": ref_c}
+        for prec in ("fp64", "fp32"):
+            buf = np.zeros((N, J), dtype=np.uint64)
+            build, draw = [], []
+            for t in range(args.trials + 1):  # the first one is a warm-up
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                smp = ftl.Sampler.from_device(U, J, R, precision=prec)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                smp.KRPDrawSamples(N + 1, buf)
+                t2 = time.perf_counter()
+                smp.close()
+                if t:
+                    build.append(t1 - t0)
+                    draw.append(t2 - t1)
+            row[f"{prec}_sampling_s"] = float(np.median(draw))
+            row[f"{prec}_construction_s"] = float(np.median(build))
+            row[f"{prec}_samples_per_s"] = J / row[f"{prec}_sampling_s"]
+            assert np.all(buf < I), "sample out of range"
+        row["sampling_ratio_fp64"] = ref_s / row["fp64_sampling_s"]
+        row["sampling_ratio_fp32"] = ref_s / row["fp32_sampling_s"]
+        log(f"sweep I=2^{I.bit_length() - 1} R={R} N={N}: draw fp64 {1e3 * row['fp64_sampling_s']:.2f} ms, "
+            f"fp32 {1e3 * row['fp32_sampling_s']:.2f} ms (reference CPU {1e3 * ref_s:.1f} ms: "
+            f"x{row['sampling_ratio_fp64']:.0f} / x{row['sampling_ratio_fp32']:.0f}); construction fp64 "
+            f"{1e3 * row['fp64_construction_s']:.1f} ms, fp32 {1e3 * row['fp32_construction_s']:.1f} ms "
+            f"(reference {1e3 * ref_c:.1f} ms)")
+        rows.append(row)
+        del U
+        torch.cuda.empty_cache()
+    head = [r for r in rows if (r["I"], r["R"], r["N"]) == (2 ** 23, 32, 3)][0]
+    line = {"metric": "KRPDrawSamples over the reference's runtime sweeps (I, R, N), j=N+1, J=50,000",
+            "value": head["fp32_samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.trials,
+            "warmup": 1, "ms_per_step": 1e3 * head["fp32_sampling_s"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 top tree levels) and f64",
+            "data": "synthetic: N(0,1) factors in HBM, the sampler's built-in Philox uniforms",
+            "config": {"workload": "benchmarks/runtime_benchmark.py sweeps: I 2^6..2^25 (R=32, N=3), "
+                                   "R 16..128 and N 2..9 (I=2^22); median of "
+                                   f"{args.trials} after 1 warm-up, host-timed with a host sample buffer"},
+            "rows": rows, "wall_s": time.perf_counter() - t_all}
+    print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------
 # mixed precision on ill-conditioned factors (--workload c3)
 # ---------------------------------------------------------------------------
 def c3_main(args):
@@ -653,10 +738,11 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
     ap.add_argument("--c3-ranks", default="16,64,128")
     ap.add_argument("--c3-steps", type=int, default=5)
-    ap.add_argument("--workload", choices=["krp", "als", "serve", "c3", "lk"], default="krp",
+    ap.add_argument("--workload", choices=["krp", "als", "serve", "c3", "lk", "sweep"], default="krp",
                     help="krp: the KRP leverage-sampling headline (default); als: sampled CP-ALS sweeps; "
                          "serve: the draw-request stream at several arrival rates; c3: FP32 vs FP64 on "
-                         "ill-conditioned factors; lk: the Larsen-Kolda sampler")
+                         "ill-conditioned factors; lk: the Larsen-Kolda sampler; sweep: the reference's "
+                         "runtime-benchmark configurations (I, R, N) against its published CPU times")
     ap.add_argument("--als-size", choices=sorted(ALS_SIZES), default="c5")
     ap.add_argument("--als-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--als-oracle", action="store_true", help="rerun the sweeps on the CPU oracle and compare fits")
@@ -691,6 +777,9 @@ def main():
         return
     if args.workload == "lk":
         lk_main(args)
+        return
+    if args.workload == "sweep":
+        sweep_main(args)
         return
 
     import torch  # device buffers only; imported before libftl_hip.so so both share one HIP runtime

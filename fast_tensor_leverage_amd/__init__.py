@@ -11,8 +11,8 @@ from .als_driver import als_prod  # noqa: F401
 from .sampler import LarsenKoldaSampler, Sampler  # noqa: F401
 from .serve import Server  # noqa: F401
 from .tensors import PyLowRank, PySparseTensor  # noqa: F401
-from ._lib import eigensolver_capped, tuning_lookup  # noqa: F401
+from ._lib import cached_memory_bytes, eigensolver_capped, release_cached_memory, tuning_lookup  # noqa: F401
 
 __all__ = ["Sampler", "LarsenKoldaSampler", "Server", "ALS", "LowRankTensor", "SparseTensor", "PyLowRank", "PySparseTensor",
-           "als_prod", "eigensolver_capped", "tuning_lookup"]
+           "als_prod", "eigensolver_capped", "tuning_lookup", "release_cached_memory", "cached_memory_bytes"]
 __version__ = "0.1.0"

@@ -24,6 +24,8 @@ SIGNATURES = {
     "ftl_version": ([], ctypes.c_char_p),
     "ftl_device_count": ([], _i32),
     "ftl_eigensolver_capped": ([_vp], _i32),
+    "ftl_device_pool_trim": ([], _i32),
+    "ftl_device_pool_held": ([_vp], _i32),
     "ftl_tuning_lookup": ([ctypes.c_char_p, _i32, _u64, _u64, _u64, ctypes.c_char_p, _u64], _i32),
     "ftl_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_create_device": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
@@ -156,6 +158,19 @@ def eigensolver_capped() -> int:
     before converging, since the library was loaded (waits for the device)."""
     out = ctypes.c_uint64()
     check(load().ftl_eigensolver_capped(ctypes.byref(out)), "eigensolver_capped")
+    return int(out.value)
+
+
+def release_cached_memory() -> None:
+    """Release the device blocks the library keeps from destroyed samplers
+    for reuse by later ones (hipFree of each)."""
+    check(load().ftl_device_pool_trim(), "release_cached_memory")
+
+
+def cached_memory_bytes() -> int:
+    """Bytes of device blocks kept from destroyed samplers."""
+    out = ctypes.c_uint64()
+    check(load().ftl_device_pool_held(ctypes.byref(out)), "cached_memory_bytes")
     return int(out.value)
 
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -27,6 +28,143 @@ thread_local std::string g_err;
 }
 void set_error(const std::string& msg) { g_err = msg; }
 const char* get_error() { return g_err.c_str(); }
+
+// Device blocks of destroyed samplers, kept for samplers built later: a
+// block of the same rounded size on the same device is handed out again, so
+// building samplers of one shape repeatedly maps and unmaps no memory
+// (hipFree of a multi-GB block takes tens of ms).  Blocks are returned only
+// by the sampler destructor, after its streams are synchronised.  At most
+// FTL_POOL_CAP_GB GiB are kept (default: a quarter of the first device's
+// memory; 0 keeps none), the oldest kept blocks released first to make room;
+// an allocation that fails releases every kept block of the device and
+// tries again.
+namespace {
+class BlockPool {
+  public:
+    static size_t rounded(size_t bytes) {
+        const size_t g = bytes >= (2u << 20) ? (2u << 20) : 256u;
+        return (bytes + g - 1) / g * g;
+    }
+
+    void* take(size_t bytes, int* dev_out) {
+        int dev = 0;
+        FTL_HIP_CHECK(hipGetDevice(&dev));
+        *dev_out = dev;
+        const size_t rb = rounded(bytes);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = free_.find({dev, rb});
+            if (it != free_.end()) {
+                void* p = it->second;
+                free_.erase(it);
+                forget_age(p);
+                held_ -= rb;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, rb) != hipSuccess) {
+            (void)hipGetLastError();
+            trim(dev);
+            FTL_HIP_CHECK(hipMalloc(&p, rb));
+        }
+        return p;
+    }
+
+    void give(void* p, size_t bytes, int dev) {
+        const size_t rb = rounded(bytes);
+        std::vector<void*> out;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            if (rb > cap()) {
+                out.push_back(p);
+            } else {
+                // oldest first (the smallest sequence numbers)
+                while (held_ + rb > cap() && !age_.empty()) {
+                    const auto old = age_.begin();
+                    const auto range = free_.equal_range(old->second.first);
+                    for (auto it = range.first; it != range.second; ++it)
+                        if (it->second == old->second.second) {
+                            free_.erase(it);
+                            break;
+                        }
+                    held_ -= old->second.first.second;
+                    out.push_back(old->second.second);
+                    age_.erase(old);
+                }
+                free_.emplace(std::make_pair(dev, rb), p);
+                age_.emplace(seq_++, std::make_pair(std::make_pair(dev, rb), p));
+                held_ += rb;
+            }
+        }
+        for (void* q : out) (void)hipFree(q);
+    }
+
+    // releases every kept block of device dev (-1: of every device)
+    void trim(int dev) {
+        std::vector<void*> out;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            for (auto it = free_.begin(); it != free_.end();) {
+                if (dev < 0 || it->first.first == dev) {
+                    out.push_back(it->second);
+                    forget_age(it->second);
+                    held_ -= it->first.second;
+                    it = free_.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        for (void* p : out) (void)hipFree(p);
+    }
+
+    size_t held() {
+        std::lock_guard<std::mutex> g(mu_);
+        return held_;
+    }
+
+  private:
+    static size_t cap() {
+        static const size_t c = [] {
+            if (const char* e = std::getenv("FTL_POOL_CAP_GB")) {
+                const double gb = std::atof(e);
+                return gb > 0 ? (size_t)(gb * 1073741824.0) : (size_t)0;
+            }
+            size_t free_b = 0, total = 0;
+            if (hipMemGetInfo(&free_b, &total) != hipSuccess) {
+                (void)hipGetLastError();
+                return (size_t)0;
+            }
+            return total / 4;
+        }();
+        return c;
+    }
+    // caller holds mu_
+    void forget_age(void* p) {
+        for (auto it = age_.begin(); it != age_.end(); ++it)
+            if (it->second.second == p) {
+                age_.erase(it);
+                return;
+            }
+    }
+    std::mutex mu_;
+    std::multimap<std::pair<int, size_t>, void*> free_;
+    // kept blocks by the order they were kept in
+    std::map<uint64_t, std::pair<std::pair<int, size_t>, void*>> age_;
+    uint64_t seq_ = 0;
+    size_t held_ = 0;
+};
+
+// never destroyed: samplers may be destroyed by static destructors at exit
+BlockPool& block_pool() {
+    static BlockPool* p = new BlockPool;
+    return *p;
+}
+}  // namespace
+
+void device_pool_trim() { block_pool().trim(-1); }
+size_t device_pool_held() { return block_pool().held(); }
 
 TuningTable parse_tuning(const std::string& text) {
     TuningTable t;
@@ -305,6 +443,8 @@ struct Sampler {
     }
     size_t bytes_held = 0;
     std::vector<std::pair<void*, size_t>> allocs;
+    // device of the blocks taken from the block pool (hipMalloc otherwise)
+    int pool_dev = -1;
     // optional sub-allocator (a serving arena); hipMalloc / hipFree otherwise
     DeviceAllocator* allocator = nullptr;
     bool own_stream = true;
@@ -317,7 +457,7 @@ struct Sampler {
             p = allocator->alloc(bytes ? bytes : 16);
             if (!p) throw Error{FTL_ENOMEM, "serving arena exhausted"};
         } else {
-            FTL_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+            p = block_pool().take(bytes ? bytes : 16, &pool_dev);
         }
         allocs.push_back({p, bytes ? bytes : 16});
         bytes_held += bytes;
@@ -327,6 +467,12 @@ struct Sampler {
     void release_block(void* p, size_t bytes) {
         if (allocator) allocator->release(p, bytes);
         else (void)hipFree(p);
+    }
+
+    // at destruction, once the handle's streams are idle: back to the pool
+    void retire_block(void* p, size_t bytes) {
+        if (allocator) allocator->release(p, bytes);
+        else block_pool().give(p, bytes, pool_dev);
     }
 
     void dfree(void* p, size_t bytes) {
@@ -352,7 +498,7 @@ struct Sampler {
         }
         if (trees_ev) (void)hipEventDestroy(trees_ev);
         if (cm_stream) (void)hipStreamDestroy(cm_stream);
-        for (auto& a : allocs) release_block(a.first, a.second);
+        for (auto& a : allocs) retire_block(a.first, a.second);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& q : pending)
@@ -1369,6 +1515,17 @@ int ftl_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int ftl_device_pool_trim(void) {
+    return guarded([&] { ftl::device_pool_trim(); });
+}
+
+int ftl_device_pool_held(uint64_t* bytes) {
+    return guarded([&] {
+        FTL_REQUIRE(bytes, "null argument");
+        *bytes = (uint64_t)ftl::device_pool_held();
+    });
 }
 
 int ftl_tuning_lookup(const char* table_text, int precision, uint64_t R, uint64_t depth, uint64_t J,

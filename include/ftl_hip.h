@@ -67,6 +67,18 @@ int ftl_device_count(void);
 int ftl_tuning_lookup(const char* table_text, int precision, uint64_t R, uint64_t depth, uint64_t J,
                       char* out, uint64_t capacity);
 
+/* Device blocks kept by the library (no counterpart in the reference): the
+ * device memory of a destroyed sampler is kept, up to FTL_POOL_CAP_GB GiB
+ * (environment; default a quarter of the device's memory; 0 keeps none; the
+ * oldest kept blocks are released first to make room), and handed to
+ * samplers built later with blocks of the same rounded size, so that
+ * rebuilding samplers of one shape maps no new memory.
+ * ftl_device_pool_trim releases every kept block (hipFree);
+ * ftl_device_pool_held reports their bytes.  An allocation of the library
+ * that fails first releases the kept blocks of its device. */
+int ftl_device_pool_trim(void);
+int ftl_device_pool_held(uint64_t* bytes);
+
 /* Eigensolves (computeM, the pseudo-inverses of the ALS and Larsen-Kolda
  * Grams) on the current device that stopped at the 40-sweep cap of the
  * cyclic Jacobi iteration before converging, since the library was loaded.

@@ -125,3 +125,29 @@ def test_file_readers_feed_the_device_tensor(ftl, tmp_path):
                     (ftl.PySparseTensor.from_tns(str(tns)), seen)):
         assert t.nnz == len(vals) and list(t.dims) == want
         np.testing.assert_allclose(t.ten.get_normsq(), float(np.sum(vals ** 2)), rtol=1e-13)
+
+
+def test_destroyed_sampler_memory_is_reused(ftl):
+    """A destroyed sampler's device blocks are kept and handed to the next
+    sampler of the same shape, whose draws are those of the first; releasing
+    the kept blocks empties the pool."""
+    print("[gpu] block pool", flush=True)
+    ftl.release_cached_memory()
+    assert ftl.cached_memory_bytes() == 0
+    rng = np.random.default_rng(44)
+    R, J = 24, 4000
+    U = [rng.standard_normal((I, R)) for I in (30000, 20000, 9000)]
+    draws = rng.random(6 * J)
+    out = []
+    for prec in ("fp32", "fp32", "fp64"):
+        s = ftl.Sampler(U, J, R, "efficient", precision=prec)
+        held = s.info()["device_bytes"]
+        kept_before = ftl.cached_memory_bytes()
+        smp = np.zeros((3, J), dtype=np.uint64)
+        s.KRPDrawSamples(3, smp, draws)
+        out.append(smp)
+        s.close()
+        assert ftl.cached_memory_bytes() >= kept_before + held
+    np.testing.assert_array_equal(out[0], out[1])
+    ftl.release_cached_memory()
+    assert ftl.cached_memory_bytes() == 0
