@@ -905,7 +905,8 @@ def main():
         "trials": {p: {"calls": len(results[p]["trial_s"]),
                        "median_samples_per_s": J / float(np.median(results[p]["trial_s"])),
                        "best_samples_per_s": J / float(np.min(results[p]["trial_s"])),
-                       "trial_ms": [1e3 * t for t in results[p]["trial_s"]]} for p in ("fp32", "fp64")},
+                       "trial_ms": [1e3 * t for t in results[p]["trial_s"]]} for p in ("fp32", "fp64")}
+        if args.trials > 0 else None,
         "cpu_baseline": cpu,
         "fp64_gpu_samples_per_s": v64,
         "fp32_over_fp64": value / v64,
