@@ -11,8 +11,10 @@ from .als_driver import als_prod  # noqa: F401
 from .sampler import LarsenKoldaSampler, Sampler  # noqa: F401
 from .serve import Server  # noqa: F401
 from .tensors import PyLowRank, PySparseTensor  # noqa: F401
-from ._lib import cached_memory_bytes, eigensolver_capped, release_cached_memory, tuning_lookup  # noqa: F401
+from ._lib import (cached_memory_bytes, cached_memory_limit, eigensolver_capped, release_cached_memory,  # noqa: F401
+                   set_cached_memory_limit, tuning_lookup)
 
 __all__ = ["Sampler", "LarsenKoldaSampler", "Server", "ALS", "LowRankTensor", "SparseTensor", "PyLowRank", "PySparseTensor",
-           "als_prod", "eigensolver_capped", "tuning_lookup", "release_cached_memory", "cached_memory_bytes"]
+           "als_prod", "eigensolver_capped", "tuning_lookup", "release_cached_memory", "cached_memory_bytes",
+           "set_cached_memory_limit", "cached_memory_limit"]
 __version__ = "0.1.0"

@@ -26,6 +26,8 @@ SIGNATURES = {
     "ftl_eigensolver_capped": ([_vp], _i32),
     "ftl_device_pool_trim": ([], _i32),
     "ftl_device_pool_held": ([_vp], _i32),
+    "ftl_device_pool_set_cap": ([_u64], _i32),
+    "ftl_device_pool_cap": ([_vp], _i32),
     "ftl_tuning_lookup": ([ctypes.c_char_p, _i32, _u64, _u64, _u64, ctypes.c_char_p, _u64], _i32),
     "ftl_sampler_create": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
     "ftl_sampler_create_device": ([_u32, _vp, _u64, _u64, _vp, _i32, _vp], _i32),
@@ -165,6 +167,18 @@ def release_cached_memory() -> None:
     """Release the device blocks the library keeps from destroyed samplers
     for reuse by later ones (hipFree of each)."""
     check(load().ftl_device_pool_trim(), "release_cached_memory")
+
+
+def set_cached_memory_limit(nbytes: int) -> None:
+    """Most bytes of device blocks kept from destroyed samplers (0: none);
+    blocks beyond a lowered limit are released, the oldest first."""
+    check(load().ftl_device_pool_set_cap(int(nbytes)), "set_cached_memory_limit")
+
+
+def cached_memory_limit() -> int:
+    out = ctypes.c_uint64()
+    check(load().ftl_device_pool_cap(ctypes.byref(out)), "cached_memory_limit")
+    return int(out.value)
 
 
 def cached_memory_bytes() -> int:

@@ -76,22 +76,11 @@ class BlockPool {
         std::vector<void*> out;
         {
             std::lock_guard<std::mutex> g(mu_);
-            if (rb > cap()) {
+            const size_t c = cap_locked();
+            if (rb > c) {
                 out.push_back(p);
             } else {
-                // oldest first (the smallest sequence numbers)
-                while (held_ + rb > cap() && !age_.empty()) {
-                    const auto old = age_.begin();
-                    const auto range = free_.equal_range(old->second.first);
-                    for (auto it = range.first; it != range.second; ++it)
-                        if (it->second == old->second.second) {
-                            free_.erase(it);
-                            break;
-                        }
-                    held_ -= old->second.first.second;
-                    out.push_back(old->second.second);
-                    age_.erase(old);
-                }
+                evict_to(c - rb, out);  // the oldest kept blocks first
                 free_.emplace(std::make_pair(dev, rb), p);
                 age_.emplace(seq_++, std::make_pair(std::make_pair(dev, rb), p));
                 held_ += rb;
@@ -124,21 +113,57 @@ class BlockPool {
         return held_;
     }
 
+    // a new cap; kept blocks beyond it are released, the oldest first
+    void set_cap(size_t bytes) {
+        std::vector<void*> out;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            cap_ = bytes;
+            cap_set_ = true;
+            evict_to(cap_, out);
+        }
+        for (void* q : out) (void)hipFree(q);
+    }
+
+    size_t cap() {
+        std::lock_guard<std::mutex> g(mu_);
+        return cap_locked();
+    }
+
   private:
-    static size_t cap() {
-        static const size_t c = [] {
+    // caller holds mu_
+    size_t cap_locked() {
+        if (!cap_set_) {
+            cap_set_ = true;
             if (const char* e = std::getenv("FTL_POOL_CAP_GB")) {
                 const double gb = std::atof(e);
-                return gb > 0 ? (size_t)(gb * 1073741824.0) : (size_t)0;
+                cap_ = gb > 0 ? (size_t)(gb * 1073741824.0) : (size_t)0;
+            } else {
+                size_t free_b = 0, total = 0;
+                if (hipMemGetInfo(&free_b, &total) != hipSuccess) {
+                    (void)hipGetLastError();
+                    total = 0;
+                }
+                cap_ = total / 4;
             }
-            size_t free_b = 0, total = 0;
-            if (hipMemGetInfo(&free_b, &total) != hipSuccess) {
-                (void)hipGetLastError();
-                return (size_t)0;
-            }
-            return total / 4;
-        }();
-        return c;
+        }
+        return cap_;
+    }
+    // caller holds mu_: releases the oldest kept blocks (into `out`) until at
+    // most `limit` bytes are kept
+    void evict_to(size_t limit, std::vector<void*>& out) {
+        while (held_ > limit && !age_.empty()) {
+            const auto old = age_.begin();
+            const auto range = free_.equal_range(old->second.first);
+            for (auto it = range.first; it != range.second; ++it)
+                if (it->second == old->second.second) {
+                    free_.erase(it);
+                    break;
+                }
+            held_ -= old->second.first.second;
+            out.push_back(old->second.second);
+            age_.erase(old);
+        }
     }
     // caller holds mu_
     void forget_age(void* p) {
@@ -154,6 +179,8 @@ class BlockPool {
     std::map<uint64_t, std::pair<std::pair<int, size_t>, void*>> age_;
     uint64_t seq_ = 0;
     size_t held_ = 0;
+    size_t cap_ = 0;
+    bool cap_set_ = false;
 };
 
 // never destroyed: samplers may be destroyed by static destructors at exit
@@ -165,6 +192,8 @@ BlockPool& block_pool() {
 
 void device_pool_trim() { block_pool().trim(-1); }
 size_t device_pool_held() { return block_pool().held(); }
+void device_pool_set_cap(size_t bytes) { block_pool().set_cap(bytes); }
+size_t device_pool_cap() { return block_pool().cap(); }
 
 TuningTable parse_tuning(const std::string& text) {
     TuningTable t;
@@ -1525,6 +1554,17 @@ int ftl_device_pool_held(uint64_t* bytes) {
     return guarded([&] {
         FTL_REQUIRE(bytes, "null argument");
         *bytes = (uint64_t)ftl::device_pool_held();
+    });
+}
+
+int ftl_device_pool_set_cap(uint64_t bytes) {
+    return guarded([&] { ftl::device_pool_set_cap((size_t)bytes); });
+}
+
+int ftl_device_pool_cap(uint64_t* bytes) {
+    return guarded([&] {
+        FTL_REQUIRE(bytes, "null argument");
+        *bytes = (uint64_t)ftl::device_pool_cap();
     });
 }
 

@@ -78,6 +78,10 @@ int ftl_tuning_lookup(const char* table_text, int precision, uint64_t R, uint64_
  * that fails first releases the kept blocks of its device. */
 int ftl_device_pool_trim(void);
 int ftl_device_pool_held(uint64_t* bytes);
+/* The most bytes kept (replaces FTL_POOL_CAP_GB; 0 keeps none); kept blocks
+ * beyond a lowered cap are released, the oldest first. */
+int ftl_device_pool_set_cap(uint64_t bytes);
+int ftl_device_pool_cap(uint64_t* bytes);
 
 /* Eigensolves (computeM, the pseudo-inverses of the ALS and Larsen-Kolda
  * Grams) on the current device that stopped at the 40-sweep cap of the

@@ -151,3 +151,29 @@ def test_destroyed_sampler_memory_is_reused(ftl):
     np.testing.assert_array_equal(out[0], out[1])
     ftl.release_cached_memory()
     assert ftl.cached_memory_bytes() == 0
+
+
+def test_block_pool_limit_evicts_the_oldest(ftl):
+    """Under a lowered limit the pool keeps at most that many bytes, the
+    blocks of the most recently destroyed sampler first; a limit of 0 keeps
+    nothing."""
+    print("[gpu] block pool limit", flush=True)
+    old = ftl.cached_memory_limit()
+    try:
+        ftl.release_cached_memory()
+        limit = 48 << 20
+        ftl.set_cached_memory_limit(limit)
+        assert ftl.cached_memory_limit() == limit
+        rng = np.random.default_rng(45)
+        for I in (40000, 60000, 80000):  # about 10, 15 and 20 MB of blocks each
+            U = [rng.standard_normal((I, 16)) for _ in range(3)]
+            s = ftl.Sampler(U, 2000, 16, "efficient", precision="fp64")
+            s.close()
+            assert 0 < ftl.cached_memory_bytes() <= limit
+        ftl.set_cached_memory_limit(0)
+        assert ftl.cached_memory_bytes() == 0
+        s = ftl.Sampler(U, 2000, 16, "efficient", precision="fp64")
+        s.close()
+        assert ftl.cached_memory_bytes() == 0
+    finally:
+        ftl.set_cached_memory_limit(old)
