@@ -104,7 +104,16 @@ ALS_SIZES = {
     # dims, nnz, R, J, sweeps
     "c5": ((2 ** 22, 2 ** 21, 2 ** 17), 200_000_000, 25, 2 ** 16, 10),
     "small": ((2 ** 16, 2 ** 15, 2 ** 12), 2_000_000, 25, 2 ** 16, 10),
+    # the shapes and nonzero counts of the FROSTT tensors of the reference's
+    # STS-CP timings, filled with synthetic Zipf data (the real tensors are
+    # not available here)
+    "uber": ((183, 24, 1140, 1717), 3_309_490, 25, 2 ** 16, 10),
+    "enron": ((6066, 5699, 244268, 1176), 54_202_099, 25, 2 ** 16, 10),
+    "nell2": ((12092, 9184, 28818), 76_879_419, 25, 2 ** 16, 10),
 }
+# the reference's published STS-CP seconds per sweep on the real tensors (R=25,
+# J=2^16, 128 threads of a Perlmutter CPU node; plotting/generate_images.ipynb:1230-1234)
+ALS_REFERENCE_S_PER_SWEEP = {"uber": 0.224, "enron": 0.462, "nell2": 0.570}
 
 
 def als_main(args):
@@ -214,6 +223,8 @@ def als_main(args):
         "sparse_construction_s": t_build,
         "hash_construction_s": t_hash,
         "eigensolves_capped": ftl.eigensolver_capped(),
+        # published on the real tensor of this shape (different data, different machine)
+        "reference_real_tensor_s_per_sweep": ALS_REFERENCE_S_PER_SWEEP.get(args.als_size),
         "generation_s": t_gen,
     }
     if args.exact_sweeps:

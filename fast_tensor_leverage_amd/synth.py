@@ -15,25 +15,29 @@ def zipf_coords(torch, n, size, s, gen, dev):
 
 
 def zipf_sparse(torch, dims, nnz, R, dev, seed=2024, exponent=1.1, noise=0.01, chunk=10_000_000):
-    """A 3-way sparse tensor on the device: ``nnz`` distinct coordinates, each
-    mode's index drawn from a truncated Zipf(``exponent``) law (int32, nnz x 3),
-    values = the rank-R CP model of N(0,1) factors at those coordinates plus
-    ``noise`` x std Gaussian noise (FP64).  Returns (idx, vals)."""
-    if len(dims) != 3:
-        raise ValueError("zipf_sparse builds 3-way tensors")
+    """A sparse tensor of ``len(dims)`` modes on the device: ``nnz`` distinct
+    coordinates, each mode's index drawn from a truncated Zipf(``exponent``)
+    law (int32, nnz x N), values = the rank-R CP model of N(0,1) factors at
+    those coordinates plus ``noise`` x std Gaussian noise (FP64).  The
+    coordinates are packed into one 63-bit key per nonzero for the
+    deduplication.  Returns (idx, vals)."""
+    N = len(dims)
+    bits = [(d - 1).bit_length() for d in dims]
+    if sum(bits) > 63:
+        raise ValueError("the coordinates of one nonzero must pack into 63 bits")
+    shift = [sum(bits[k + 1:]) for k in range(N)]
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-    b1, b2 = (dims[1] - 1).bit_length(), (dims[2] - 1).bit_length()
     keys = torch.empty(0, dtype=torch.int64, device=dev)
     while keys.numel() < nnz:
         m = max(int(1.25 * (nnz - keys.numel())), 1_000_000)
-        k = (zipf_coords(torch, dims[0], m, exponent, gen, dev) << (b1 + b2)) | \
-            (zipf_coords(torch, dims[1], m, exponent, gen, dev) << b2) | \
-            zipf_coords(torch, dims[2], m, exponent, gen, dev)
+        k = zipf_coords(torch, dims[0], m, exponent, gen, dev) << shift[0]
+        for q in range(1, N):
+            k |= zipf_coords(torch, dims[q], m, exponent, gen, dev) << shift[q]
         keys = torch.unique(torch.cat([keys, k]))
         del k
     keys = keys[torch.randperm(keys.numel(), generator=gen, device=dev)[:nnz]]
-    idx = torch.stack([keys >> (b1 + b2), (keys >> b2) & ((1 << b1) - 1), keys & ((1 << b2) - 1)], dim=1)
+    idx = torch.stack([(keys >> shift[q]) & ((1 << bits[q]) - 1) for q in range(N)], dim=1)
     del keys
     idx = idx.to(torch.int32).contiguous()
     A = [torch.randn((d, R), generator=gen, device=dev, dtype=torch.float64) for d in dims]
@@ -41,7 +45,7 @@ def zipf_sparse(torch, dims, nnz, R, dev, seed=2024, exponent=1.1, noise=0.01, c
     for s0 in range(0, nnz, chunk):
         s1 = min(nnz, s0 + chunk)
         p = A[0][idx[s0:s1, 0].long()]
-        for k in range(1, 3):
+        for k in range(1, N):
             p *= A[k][idx[s0:s1, k].long()]
         vals[s0:s1] = p.sum(dim=1)
     vals += noise * vals.std() * torch.randn(nnz, generator=gen, device=dev, dtype=torch.float64)
