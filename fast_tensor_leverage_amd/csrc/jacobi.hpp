@@ -100,7 +100,7 @@ static inline uint64_t jacobi_capped_read() {
 }
 
 template <int BS>
-__device__ bool jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
+__device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double* lam, double* Vout, JacobiShared& js,
                              double* diag_tmp) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
@@ -110,14 +110,33 @@ __device__ bool jacobi_block(double* A, double* V, int R, double* lam, double* V
     for (int e = tid; e < R * R; e += BS) V[(e / R) * LD + e % R] = (e / R == e % R) ? 1.0 : 0.0;
     __syncthreads();
     bool converged = false;
+    const int nblk = npairs * npairs, nv = R * npairs;
+    // When the block holds at most one 2x2-block item and two V items per
+    // thread (R <= 64 at 1024 threads), their indices are the same in every
+    // round and are computed once; the thread then issues the loads of all
+    // its items before any arithmetic, one dependent LDS chain per round.
+    const bool fused = nblk <= BS && nv <= 2 * BS;
+    // (row or pair fk, pair fk2): the block item and the first V item; (row
+    // fi1, pair fk1): the second V item.  Clamped to 0 past the item counts so
+    // that every js read stays inside its 64 entries (fk < R <= 64 when fused).
+    const int fk = tid < nv ? tid / npairs : 0, fk2 = tid < nv ? tid - (tid / npairs) * npairs : 0;
+    const int fi1 = tid + BS < nv ? (tid + BS) / npairs : 0;
+    const int fk1 = tid + BS < nv ? tid + BS - ((tid + BS) / npairs) * npairs : 0;
     for (int sweep = 0; sweep < kJacobiMaxSweeps; sweep++) {
         int any = 0;
+        // circle-method positions of pair tid (jacobi_pair), advanced by one
+        // (mod n2 - 1) per round instead of recomputed by division
+        int pa = tid - 1, pb = n2 - 2 - tid;
         for (int round = 0; round + 1 < n2; round++) {
             // rotations of this round, each computed once from the current A
             if (tid < npairs) {
-                int p, q;
+                int p = tid == 0 ? 0 : 1 + pa, q = 1 + pb;  // tid < n2/2: q's position is never the fixed one
+                if (p > q) {
+                    const int t = p;
+                    p = q;
+                    q = t;
+                }
                 double c, s;
-                jacobi_pair(tid, round, n2, p, q);
                 const bool act = jacobi_rotation(A, LD, R, p, q, c, s);
                 js.p[tid] = p;
                 js.q[tid] = q;
@@ -126,7 +145,67 @@ __device__ bool jacobi_block(double* A, double* V, int R, double* lam, double* V
                 js.act[tid] = act;
                 any |= act;
             }
+            pa = pa + 1 == n2 - 1 ? 0 : pa + 1;
+            pb = pb + 1 == n2 - 1 ? 0 : pb + 1;
             __syncthreads();
+            if (fused) {
+                // J^T A J on block (fk, fk2) and V J on rows fk and fi1
+                const bool bon = tid < nblk, v0on = tid < nv, v1on = tid + BS < nv;
+                const bool act1 = bon && js.act[fk], act2 = bon && js.act[fk2];
+                const bool a0 = v0on && js.act[fk2], a1 = v1on && js.act[fk1];
+                const int p = js.p[fk], q = js.q[fk], p2 = js.p[fk2], q2 = js.q[fk2];
+                const int p3 = js.p[fk1], q3 = js.q[fk1];
+                const bool blk = act1 || act2;
+                const bool qv = q < R, q2v = q2 < R;
+                double app2 = 0.0, apq2 = 0.0, aqp2 = 0.0, aqq2 = 0.0, vp0 = 0.0, vq0 = 0.0, vp1 = 0.0, vq1 = 0.0;
+                if (blk) {
+                    app2 = A[p * LD + p2];
+                    if (q2v) apq2 = A[p * LD + q2];
+                    if (qv) aqp2 = A[q * LD + p2];
+                    if (qv && q2v) aqq2 = A[q * LD + q2];
+                }
+                if (a0) {
+                    vp0 = V[fk * LD + p2];
+                    vq0 = V[fk * LD + q2];
+                }
+                if (a1) {
+                    vp1 = V[fi1 * LD + p3];
+                    vq1 = V[fi1 * LD + q3];
+                }
+                if (blk) {
+                    const double c = js.c[fk], s = js.s[fk], c2 = js.c[fk2], s2 = js.s[fk2];
+                    double rpp2 = app2, rpq2 = apq2, rqp2 = aqp2, rqq2 = aqq2;
+                    if (act1) {
+                        rpp2 = c * app2 - s * aqp2;
+                        rqp2 = s * app2 + c * aqp2;
+                        rpq2 = c * apq2 - s * aqq2;
+                        rqq2 = s * apq2 + c * aqq2;
+                    }
+                    double npp2 = rpp2, npq2 = rpq2, nqp2 = rqp2, nqq2 = rqq2;
+                    if (act2) {
+                        npp2 = c2 * rpp2 - s2 * rpq2;
+                        npq2 = s2 * rpp2 + c2 * rpq2;
+                        nqp2 = c2 * rqp2 - s2 * rqq2;
+                        nqq2 = s2 * rqp2 + c2 * rqq2;
+                    }
+                    A[p * LD + p2] = npp2;
+                    if (q2v) A[p * LD + q2] = npq2;
+                    if (qv) A[q * LD + p2] = nqp2;
+                    if (qv && q2v) A[q * LD + q2] = nqq2;
+                }
+                if (a0) {
+                    const double c = js.c[fk2], s = js.s[fk2];
+                    V[fk * LD + p2] = c * vp0 - s * vq0;
+                    V[fk * LD + q2] = s * vp0 + c * vq0;
+                }
+                if (a1) {
+                    const double c = js.c[fk1], s = js.s[fk1];
+                    V[fi1 * LD + p3] = c * vp1 - s * vq1;
+                    V[fi1 * LD + q3] = s * vp1 + c * vq1;
+                }
+                __syncthreads();
+                continue;
+            }
             // J^T A J in place: the 2x2 block where row pair k meets column
             // pair k2 belongs to one thread, which applies the row rotation
             // and then the column rotation (the operation order of applying
