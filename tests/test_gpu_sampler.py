@@ -29,6 +29,10 @@ CASES = [
     ("n5_r8", [50, 60, 70, 80, 90], 8, 5_000, 2),
     ("n1", [1000], 8, 1_000, 1),
     ("tiny_rows", [1, 5, 7], 4, 1_000, 3),
+    ("r1", [200, 300, 100], 1, 2_000, 4),
+    ("r33_excl2", [600, 500, 400], 33, 3_000, 2),
+    ("r63", [700, 600, 500], 63, 2_000, 4),
+    ("r65_excl1", [700, 600, 500], 65, 2_000, 1),
 ]
 
 
@@ -106,6 +110,20 @@ def test_fp64_parity_ill_conditioned(ftl, oracle, cid, dims, R, J, j):
     print(f"[gpu] fp64 parity case={cid} dims={dims} R={R} J={J} j={j}", flush=True)
     U = _ill_conditioned(dims, R, seed=zlib.crc32(cid.encode()))
     _check_fp64_parity(ftl, oracle, cid, U, R, J, j, check_e_rows=False)
+
+
+# Eigensolver block sizes on either side of the rank where a thread's
+# rotation items stop fitting one 2x2 block and two V entries (R <= 32 at 256
+# threads, R <= 44 at 512, R <= 64 at 1024): (R, threads, j)
+JACOBI_CASES = [(32, 256, 4), (33, 256, 0), (44, 512, 4), (45, 512, 2), (63, 1024, 1), (64, 1024, 4), (65, 1024, 4)]
+
+
+@pytest.mark.parametrize("R,bs,j", JACOBI_CASES, ids=[f"r{c[0]}_bs{c[1]}" for c in JACOBI_CASES])
+def test_fp64_parity_jacobi_block_sizes(ftl, oracle, R, bs, j):
+    cid = f"jacobi_r{R}_bs{bs}"
+    print(f"[gpu] fp64 parity case={cid} R={R} jacobi_threads={bs} j={j}", flush=True)
+    U = _factors([600, 500, 400], R, seed=zlib.crc32(cid.encode()))
+    _check_fp64_parity(ftl, oracle, cid, U, R, 2_000, j, check_e_rows=True, options={"jacobi_threads": bs})
 
 
 # one-sample-per-block draw kernel (draw_kernel=2) at R > 32: id, dims, R, J,
