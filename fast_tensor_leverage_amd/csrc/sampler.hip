@@ -396,6 +396,10 @@ struct Sampler {
     // until that draw has read the old factors and trees.
     hipEvent_t handle_ev = nullptr, foreign_ev = nullptr;
     bool foreign_pending = false;
+    // some draw ran on a caller's stream: the destructor then waits for the
+    // whole device before its blocks go back to the pool (foreign_ev covers
+    // only the last such stream)
+    bool foreign_used = false;
 
     void wait_foreign_draws() {
         if (!foreign_pending) return;
@@ -517,6 +521,7 @@ struct Sampler {
 
     ~Sampler() {
         if (foreign_pending) (void)hipEventSynchronize(foreign_ev);
+        if (foreign_used) (void)hipDeviceSynchronize();
         if (handle_ev) (void)hipEventDestroy(handle_ev);
         if (foreign_ev) (void)hipEventDestroy(foreign_ev);
         if (cm_stream) (void)hipStreamSynchronize(cm_stream);
@@ -1293,6 +1298,7 @@ struct Sampler {
         if (foreign) {
             FTL_HIP_CHECK(hipEventRecord(foreign_ev, st));
             foreign_pending = true;
+            foreign_used = true;
         }
         if (timing) {
             FTL_HIP_CHECK(hipEventRecord(evs[4], st));
