@@ -14,7 +14,9 @@ namespace ftl {
 // then applies J^T A J in place: each thread owns the 2x2 blocks where a row
 // pair meets a column pair and applies the row rotation, then the column
 // rotation (the same operations, in the same order, as applying all row
-// rotations and then all column rotations); V is rotated in place.  Rotation
+// rotations and then all column rotations); V is rotated in place, stored
+// transposed (buffer row v holds eigenvector column v, so the entries one
+// rotation touches are contiguous).  Rotation
 // annihilating A[p][q] (Golub & Van Loan Alg. 8.5.2), skipped when
 // |A[p][q]| <= eps*sqrt(|A[p][p] A[q][q]|).  Products and sums are rounded
 // individually (no contraction).  On return lam is ascending and Vout column
@@ -116,12 +118,14 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
     // round and are computed once; the thread then issues the loads of all
     // its items before any arithmetic, one dependent LDS chain per round.
     const bool fused = nblk <= BS && nv <= 2 * BS;
-    // (row or pair fk, pair fk2): the block item and the first V item; (row
-    // fi1, pair fk1): the second V item.  Clamped to 0 past the item counts so
-    // that every js read stays inside its 64 entries (fk < R <= 64 when fused).
-    const int fk = tid < nv ? tid / npairs : 0, fk2 = tid < nv ? tid - (tid / npairs) * npairs : 0;
-    const int fi1 = tid + BS < nv ? (tid + BS) / npairs : 0;
-    const int fk1 = tid + BS < nv ? tid + BS - ((tid + BS) / npairs) * npairs : 0;
+    // (pair fk, pair fk2): the block item; (pair fk0, row fi0) and (pair fk1,
+    // row fi1): the V items.  Clamped to 0 past the item counts so that every
+    // js read stays inside its 64 entries.
+    const int fk = tid < nblk ? tid / npairs : 0, fk2 = tid < nblk ? tid - (tid / npairs) * npairs : 0;
+    // V entries: item it is row it % R of pair it / R (V is stored
+    // transposed, so consecutive lanes touch consecutive entries of a column)
+    const int fk0 = tid < nv ? tid / R : 0, fi0 = tid < nv ? tid - (tid / R) * R : 0;
+    const int fk1 = tid + BS < nv ? (tid + BS) / R : 0, fi1 = tid + BS < nv ? tid + BS - ((tid + BS) / R) * R : 0;
     for (int sweep = 0; sweep < kJacobiMaxSweeps; sweep++) {
         int any = 0;
         // circle-method positions of pair tid (jacobi_pair), advanced by one
@@ -149,12 +153,13 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
             pb = pb + 1 == n2 - 1 ? 0 : pb + 1;
             __syncthreads();
             if (fused) {
-                // J^T A J on block (fk, fk2) and V J on rows fk and fi1
+                // J^T A J on block (fk, fk2) and V J on entries (fi0, pair fk0)
+                // and (fi1, pair fk1)
                 const bool bon = tid < nblk, v0on = tid < nv, v1on = tid + BS < nv;
                 const bool act1 = bon && js.act[fk], act2 = bon && js.act[fk2];
-                const bool a0 = v0on && js.act[fk2], a1 = v1on && js.act[fk1];
+                const bool a0 = v0on && js.act[fk0], a1 = v1on && js.act[fk1];
                 const int p = js.p[fk], q = js.q[fk], p2 = js.p[fk2], q2 = js.q[fk2];
-                const int p3 = js.p[fk1], q3 = js.q[fk1];
+                const int p0 = js.p[fk0], q0 = js.q[fk0], p3 = js.p[fk1], q3 = js.q[fk1];
                 const bool blk = act1 || act2;
                 const bool qv = q < R, q2v = q2 < R;
                 double app2 = 0.0, apq2 = 0.0, aqp2 = 0.0, aqq2 = 0.0, vp0 = 0.0, vq0 = 0.0, vp1 = 0.0, vq1 = 0.0;
@@ -165,12 +170,12 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
                     if (qv && q2v) aqq2 = A[q * LD + q2];
                 }
                 if (a0) {
-                    vp0 = V[fk * LD + p2];
-                    vq0 = V[fk * LD + q2];
+                    vp0 = V[p0 * LD + fi0];
+                    vq0 = V[q0 * LD + fi0];
                 }
                 if (a1) {
-                    vp1 = V[fi1 * LD + p3];
-                    vq1 = V[fi1 * LD + q3];
+                    vp1 = V[p3 * LD + fi1];
+                    vq1 = V[q3 * LD + fi1];
                 }
                 if (blk) {
                     const double c = js.c[fk], s = js.s[fk], c2 = js.c[fk2], s2 = js.s[fk2];
@@ -194,14 +199,14 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
                     if (qv && q2v) A[q * LD + q2] = nqq2;
                 }
                 if (a0) {
-                    const double c = js.c[fk2], s = js.s[fk2];
-                    V[fk * LD + p2] = c * vp0 - s * vq0;
-                    V[fk * LD + q2] = s * vp0 + c * vq0;
+                    const double c = js.c[fk0], s = js.s[fk0];
+                    V[p0 * LD + fi0] = c * vp0 - s * vq0;
+                    V[q0 * LD + fi0] = s * vp0 + c * vq0;
                 }
                 if (a1) {
                     const double c = js.c[fk1], s = js.s[fk1];
-                    V[fi1 * LD + p3] = c * vp1 - s * vq1;
-                    V[fi1 * LD + q3] = s * vp1 + c * vq1;
+                    V[p3 * LD + fi1] = c * vp1 - s * vq1;
+                    V[q3 * LD + fi1] = s * vp1 + c * vq1;
                 }
                 __syncthreads();
                 continue;
@@ -242,15 +247,15 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
                 if (qv) A[q * LD + p2] = nqp2;
                 if (qv && q2v) A[q * LD + q2] = nqq2;
             }
-            // V J
+            // V J, on the transposed store: entry (i, pair k2), i fastest
             for (int it = tid; it < R * npairs; it += BS) {
-                const int i = it / npairs, k2 = it - (it / npairs) * npairs;
+                const int k2 = it / R, i = it - (it / R) * R;
                 if (!js.act[k2]) continue;
                 const int p = js.p[k2], q = js.q[k2];
                 const double c = js.c[k2], s = js.s[k2];
-                const double vp = V[i * LD + p], vq = V[i * LD + q];
-                V[i * LD + p] = c * vp - s * vq;
-                V[i * LD + q] = s * vp + c * vq;
+                const double vp = V[p * LD + i], vq = V[q * LD + i];
+                V[p * LD + i] = c * vp - s * vq;
+                V[q * LD + i] = s * vp + c * vq;
             }
             __syncthreads();
         }
@@ -287,7 +292,7 @@ __device__ __forceinline__ bool jacobi_block(double* A, double* V, int R, double
             rank += (du < d) || (du == d && u < v);
         }
         if (i == 0) lam[rank] = d;
-        Vout[i * R + rank] = V[i * LD + v];
+        Vout[i * R + rank] = V[v * LD + i];
     }
     if (!converged && tid == 0) atomicAdd(&g_jacobi_capped, 1u);
     __syncthreads();
